@@ -1,0 +1,167 @@
+"""Device-side plumbing shared by the drop-in pair counters.
+
+PyTorch-ROCm is used only for device memory and the current HIP stream; the
+counting itself is libhalopair.so (hand-written HIP kernels behind a C ABI).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _engine as E
+
+
+def require_gpu(device=None):
+    if not torch.cuda.is_available():
+        raise RuntimeError("halotools_amd pair counters need a ROCm GPU (torch.cuda.is_available() is False); "
+                           "there is no CPU fallback")
+    E.library()
+    return torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+
+
+def is_device_tensor(a):
+    return isinstance(a, torch.Tensor) and a.is_cuda
+
+
+def same_array(a, b):
+    """True when a and b are the same sample (identical object or identical view)."""
+    if a is b:
+        return True
+    if isinstance(a, np.ndarray) and isinstance(b, np.ndarray):
+        return (a.shape == b.shape and a.strides == b.strides and a.dtype == b.dtype
+                and a.__array_interface__["data"][0] == b.__array_interface__["data"][0])
+    if isinstance(a, torch.Tensor) and isinstance(b, torch.Tensor):
+        return (a.shape == b.shape and a.stride() == b.stride() and a.dtype == b.dtype
+                and a.device == b.device and a.data_ptr() == b.data_ptr())
+    return False
+
+
+def to_device_columns(cols, device):
+    """Three coordinate columns (NumPy or torch, any real dtype) -> contiguous float64 device tensors."""
+    if all(isinstance(c, torch.Tensor) for c in cols):
+        return tuple(c.to(device=device, dtype=torch.float64).contiguous() for c in cols)
+    host = np.empty((3, len(cols[0])), dtype=np.float64)
+    for k in range(3):
+        host[k] = np.asarray(cols[k])
+    t = torch.from_numpy(host).to(device, non_blocking=False)
+    return t[0], t[1], t[2]
+
+
+def to_device_weights(w, device):
+    if isinstance(w, torch.Tensor):
+        return w.to(device=device, dtype=torch.float64).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(w, dtype=np.float64)).to(device)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Result(object):
+    pass
+
+
+def run(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0, wfunc=0,
+        want_counts=True, want_sums=False, rows1=False, rows2=False, want_evals=False,
+        also_counts=False, device=None):
+    """Enqueue one request on the current stream and return the output tensors.
+
+    mocks : list of dicts with n1, n2, period (3,), pbc, cell (3,) (cell <= 0: default)
+    s1, s2 : tuples (x, y, z) of float64 device tensors, mocks concatenated
+    r2 : host float64 array of squared radial / rp edges (already squared like the reference)
+    """
+    device = torch.device(device) if device is not None else s1[0].device
+    M = len(mocks)
+    marr = (E.HpMock * M)()
+    n1_tot = n2_tot = 0
+    for m, d in enumerate(mocks):
+        marr[m].n1 = int(d["n1"])
+        marr[m].n2 = int(d["n1"] if same else d["n2"])
+        marr[m].period[:] = [float(v) for v in d["period"]]
+        marr[m].pbc = 1 if d["pbc"] else 0
+        marr[m].cell[:] = [float(v) for v in d.get("cell", (0.0, 0.0, 0.0))]
+        n1_tot += marr[m].n1
+        n2_tot += marr[m].n2
+    r2 = np.ascontiguousarray(r2, dtype=np.float64)
+    nr = len(r2)
+    if p2 is not None:
+        p2 = np.ascontiguousarray(p2, dtype=np.float64)
+    flags = 0
+    if same:
+        flags |= E.HP_SAME_SAMPLES
+    marked = kind == E.HP_MARKED_NPAIRS_3D
+    if marked:
+        if rows1:
+            flags |= E.HP_ROWS1
+        if rows2:
+            flags |= E.HP_ROWS2
+        if not want_sums:
+            flags |= E.HP_NO_SUMS
+        if also_counts:
+            flags |= E.HP_ALSO_COUNTS
+    req = E.HpRequest()
+    req.kind = kind
+    req.flags = flags
+    req.nmocks = M
+    req.wfunc = int(wfunc)
+    req.mocks = ctypes.cast(marr, ctypes.POINTER(E.HpMock))
+    req.x1, req.y1, req.z1 = [c.data_ptr() for c in s1]
+    if not same:
+        req.x2, req.y2, req.z2 = [c.data_ptr() for c in s2]
+    req.w1 = w1.data_ptr() if w1 is not None else None
+    req.w2 = w2.data_ptr() if (w2 is not None and not same) else None
+    req.nw = int(nw)
+    req.nr = nr
+    req.np = len(p2) if p2 is not None else 0
+    req.r2 = r2.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    req.p2 = p2.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if p2 is not None else None
+
+    res = Result()
+    ncount = nr * (len(p2) if kind == E.HP_NPAIRS_XY_Z else 1)
+    need_counts = (not marked) or also_counts
+    # counts and the status word share one buffer so one copy brings both back
+    cbuf = torch.empty(M * ncount + 1 if need_counts else 1, dtype=torch.int64, device=device)
+    res.status = cbuf[-1:]
+    if need_counts:
+        shape = (M, nr, len(p2)) if kind == E.HP_NPAIRS_XY_Z else (M, nr)
+        res.counts = cbuf[:-1].view(*shape)
+        req.counts = cbuf.data_ptr()
+    else:
+        res.counts = None
+    req.status = res.status.data_ptr()
+    res.sums = torch.empty((M, nr), dtype=torch.float64, device=device) if (marked and want_sums) else None
+    req.sums = res.sums.data_ptr() if res.sums is not None else None
+    res.rows1 = torch.empty((n1_tot, nr), dtype=torch.float64, device=device) if rows1 else None
+    res.rows2 = torch.empty((n2_tot, nr), dtype=torch.float64, device=device) if rows2 else None
+    req.rows1 = res.rows1.data_ptr() if rows1 else None
+    req.rows2 = res.rows2.data_ptr() if rows2 else None
+    res.evals = torch.empty(M, dtype=torch.int64, device=device) if want_evals else None
+    req.evals = res.evals.data_ptr() if want_evals else None
+
+    lib = E.library()
+    nbytes = lib.hp_workspace_size(ctypes.byref(req))
+    if nbytes == 0:
+        E.check(-1)
+    ws = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
+    req.workspace = ws.data_ptr()
+    req.workspace_bytes = int(nbytes)
+    req.stream = torch.cuda.current_stream(device).cuda_stream
+    fn = {E.HP_NPAIRS_3D: lib.hp_npairs_3d, E.HP_NPAIRS_PROJECTED: lib.hp_npairs_projected,
+          E.HP_NPAIRS_XY_Z: lib.hp_npairs_xy_z, E.HP_MARKED_NPAIRS_3D: lib.hp_marked_npairs_3d}[kind]
+    E.check(fn(ctypes.byref(req)))
+    res.workspace = ws  # keep alive until the stream has consumed it (caching allocator)
+    res.cbuf = cbuf
+    return res
+
+
+def backward_rows(rows, gsuffix, offsets):
+    """grad[p] = rows[p] . gsuffix[mock(p)] on the current stream."""
+    n = int(offsets[-1])
+    grad = torch.empty(n, dtype=torch.float64, device=rows.device)
+    offs = (ctypes.c_int64 * len(offsets))(*[int(v) for v in offsets])
+    g = gsuffix.contiguous()
+    E.check(E.library().hp_marked_backward(
+        ctypes.c_void_p(rows.data_ptr()), rows.shape[1], ctypes.c_void_p(g.data_ptr()), offs,
+        len(offsets) - 1, ctypes.c_void_p(grad.data_ptr()),
+        ctypes.c_void_p(torch.cuda.current_stream(rows.device).cuda_stream)))
+    return grad

@@ -1,0 +1,531 @@
+// capi.hip -- extern "C" entry points of libhalopair.so (include/halopair.h).
+//
+// Host-side planning mirrors what the reference does in Python before it calls
+// an engine (PC/npairs_3d.py:111-150): square the bins (rbins*rbins,
+// PC/cpairs/npairs_3d_engine.pyx:46), choose a mesh, and run the engine over
+// every cell.  The GPU mesh is its own (fine cells sized from the search
+// radius, not the reference's <= 50 cells per dimension); counts do not depend
+// on the mesh because every pair is evaluated with its minimum-image shift.
+#include <math.h>
+
+#include <cstring>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "hp_internal.h"
+
+namespace hp {
+int launch_reduce_counts_s(const uint64_t* partial, int nr, int np, const MockGeom* geom, int nmocks, int side,
+                           int64_t max_groups, uint64_t* scratch, int64_t* counts, hipStream_t s);
+int launch_reduce_sums_s(const double* partial, int nr, const MockGeom* geom, int nmocks, int side, int64_t max_groups,
+                         double* scratch, double* sums, hipStream_t s);
+int launch_reduce_evals_s(const uint64_t* pevals, const MockGeom* geom, int nmocks, int side, int64_t max_groups,
+                          uint64_t* scratch, int64_t* evals, hipStream_t s);
+int64_t reduce_scratch_elems(int64_t max_groups_per_mock, int nmocks, int nb);
+}  // namespace hp
+
+using namespace hp;
+
+static thread_local std::string g_err;
+
+static int fail(const std::string& msg) {
+    g_err = msg;
+    return -1;
+}
+
+// ------------------------------------------------------------ bin tables
+static uint32_t hiword(double d) {
+    uint64_t u;
+    std::memcpy(&u, &d, 8);
+    return (uint32_t)(u >> 32);
+}
+static double from_hiword(uint32_t h) {
+    uint64_t u = (uint64_t)h << 32;
+    double d;
+    std::memcpy(&d, &u, 8);
+    return d;
+}
+
+// Edges as the kernels see them: the reference increments counts[k] for every
+// k from K-1 down while dsq <= r2[k] (PC/cpairs/npairs_3d_engine.pyx:178-182),
+// so a pair lands in bin k iff dsq <= min(r2[k..K-1]).  The suffix minimum is
+// ascending, and the first bin of a pair is #{k : e[k] < dsq}.
+static std::vector<double> suffix_min(const double* r2, int K) {
+    std::vector<double> e((size_t)K + 2, INFINITY);
+    double run = INFINITY;
+    for (int k = K - 1; k >= 0; k--) {
+        run = std::min(run, r2[k]);
+        e[(size_t)k] = run;
+    }
+    return e;
+}
+
+struct Table {
+    BinSpec spec{};
+    std::vector<double> e;       // K + 2 entries
+    std::vector<TabEntry> t;
+};
+
+static Table make_table(const std::vector<double>& e_in, int K) {
+    Table T;
+    T.e = e_in;
+    T.spec.nbins = K;
+    double first_pos = INFINITY;
+    for (int k = 0; k < K; k++)
+        if (T.e[k] > 0.0 && T.e[k] < first_pos) first_pos = T.e[k];
+    const double last = K > 0 ? T.e[K - 1] : 0.0;
+    int shift = 20, base = 0, nb = 2;
+    if (first_pos < INFINITY && last < INFINITY) {
+        for (shift = 0; shift <= 31; shift++) {
+            base = std::max((int)(hiword(first_pos) >> shift) - 1, 0);
+            nb = (int)(hiword(last) >> shift) - base + 2;
+            if (nb <= kTableMax) break;
+        }
+    }
+    T.spec.shift = shift;
+    T.spec.base = base;
+    T.spec.basesh = (uint32_t)base << shift;
+    T.spec.nb = nb;
+    T.t.assign((size_t)nb, TabEntry{0.0, 0u, 0u});
+    auto lo = [&](int b) -> double {
+        if (b <= 0) return 0.0;
+        return from_hiword((uint32_t)((int64_t)(b + base) << shift));
+    };
+    int multi = 0;
+    for (int b = 0; b < nb; b++) {
+        const double l = lo(b);
+        const double h = (b + 1 < nb) ? lo(b + 1) : INFINITY;
+        int below = 0, inside = 0;
+        for (int k = 0; k < K; k++) {
+            if (T.e[k] < l) below++;
+            else if (T.e[k] < h || (b + 1 == nb)) inside++;
+        }
+        T.t[(size_t)b].k = (uint32_t)below;
+        T.t[(size_t)b].edge = T.e[(size_t)below];
+        if (inside > 1) multi = 1;
+    }
+    T.spec.multi = multi;
+    return T;
+}
+
+static int lookup_host(const Table& T, double d) {
+    const uint32_t hi = hiword(d);
+    const uint32_t t = hi > T.spec.basesh ? hi - T.spec.basesh : 0u;
+    const uint32_t b = std::min(t >> T.spec.shift, (uint32_t)(T.spec.nb - 1));
+    const TabEntry& e = T.t[b];
+    if (!T.spec.multi) return (int)e.k + (e.edge < d ? 1 : 0);
+    uint32_t k = e.k;
+    while (T.e[(size_t)k] < d) ++k;
+    return (int)k;
+}
+
+// ------------------------------------------------------------ plan
+struct Side {
+    bool needed = false;
+    size_t x = 0, y = 0, z = 0, w = 0, perm = 0, cs = 0;
+};
+
+struct Plan {
+    int kind = 0, flags = 0, M = 0;
+    bool same = false, marked = false, rows1 = false, rows2 = false, sums = false, counts = false;
+    int nw = 0;
+    int64_t n[2] = {0, 0};
+    int64_t ngroups[2] = {0, 0};
+    int64_t max_groups[2] = {0, 0};
+    int64_t total_cells = 0, total_keys = 0;
+    int bits_row = 1, bits_morton = 1;
+    std::vector<MockGeom> geom;
+    Table tr, tp;
+    int Kr = 0, Kp = 0, nb_count = 0;
+    // workspace layout (byte offsets)
+    size_t o_geom = 0, o_er = 0, o_ep = 0, o_tr = 0, o_tp = 0;
+    size_t o_gm[2] = {0, 0};
+    Side sorted[2][2];  // [data side][order]
+    size_t o_keys[2] = {0, 0}, o_vals[2] = {0, 0}, o_tiles = 0, o_digits = 0;
+    size_t o_pcount = 0, o_psum = 0, o_pevals = 0, o_red = 0, o_status = 0;
+    size_t total = 0;
+    std::vector<unsigned char> consts;  // bytes [0, o_gm[0])
+};
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+static int bits_for(uint64_t range) {
+    int b = 1;
+    while (b < 32 && ((uint64_t)1 << b) < range) b++;
+    return b;
+}
+
+static int make_plan(const hp_request* r, int kind, Plan& P) {
+    if (!r) return fail("null request");
+    P.kind = kind;
+    P.flags = r->flags;
+    P.M = r->nmocks;
+    if (P.M < 1) return fail("nmocks must be >= 1");
+    if (!r->mocks) return fail("mocks must not be null");
+    P.same = (r->flags & HP_SAME_SAMPLES) != 0;
+    P.marked = (kind == HP_MARKED_NPAIRS_3D);
+    P.rows1 = P.marked && (r->flags & HP_ROWS1);
+    P.rows2 = P.marked && (r->flags & HP_ROWS2);
+    P.sums = P.marked && !(r->flags & HP_NO_SUMS);
+    P.counts = !P.marked || (r->flags & HP_ALSO_COUNTS);
+    P.nw = P.marked ? r->nw : 0;
+    if (P.marked) {
+        if (r->wfunc < 1 || r->wfunc > 17) return fail("weight_func_id must be in 1..17");
+        if (P.nw < 1 || P.nw > 5) return fail("weights per point must be in 1..5");
+        if ((P.rows1 || P.rows2) && (r->wfunc != 1 || P.nw != 1))
+            return fail("per-point rows (HP_ROWS1/2) need weight_func_id 1 with one weight per point");
+    }
+    P.Kr = r->nr;
+    if (P.Kr < 1 || P.Kr > HP_MAX_BINS) return fail("number of radial bins out of range");
+    if (!r->r2) return fail("r2 must not be null");
+    if (kind == HP_NPAIRS_XY_Z || kind == HP_NPAIRS_PROJECTED) {
+        P.Kp = r->np;
+        if (P.Kp < 1 || P.Kp > HP_MAX_BINS || !r->p2) return fail("pi bins out of range");
+        if (kind == HP_NPAIRS_PROJECTED && P.Kp != 1) return fail("npairs_projected takes one pi_max");
+    }
+    if (kind == HP_NPAIRS_XY_Z && P.Kr * P.Kp > 16384) return fail("rp_bins x pi_bins exceeds 16384 cells");
+    if (P.marked && P.Kr > 128) return fail("marked counts support at most 128 bins");
+
+    std::vector<double> er = suffix_min(r->r2, P.Kr);
+    P.tr = make_table(er, P.Kr);
+    double Rxy = sqrt(er[(size_t)P.Kr - 1]);
+    double Rz = Rxy;
+    int shape = 0;
+    if (kind == HP_NPAIRS_XY_Z) {
+        std::vector<double> ep = suffix_min(r->p2, P.Kp);
+        P.tp = make_table(ep, P.Kp);
+        Rz = sqrt(ep[(size_t)P.Kp - 1]);
+        shape = 1;
+    } else if (kind == HP_NPAIRS_PROJECTED) {
+        P.tp.e.assign(3, INFINITY);
+        P.tp.e[0] = r->p2[0];
+        P.tp.spec.nbins = 1;
+        P.tp.spec.nb = 1;
+        P.tp.t.assign(1, TabEntry{INFINITY, 0u, 0u});
+        Rz = sqrt(r->p2[0]);
+        shape = 1;
+    } else {
+        P.tp.e.assign(3, INFINITY);
+        P.tp.spec.nbins = 0;
+        P.tp.spec.nb = 1;
+        P.tp.t.assign(1, TabEntry{INFINITY, 0u, 0u});
+    }
+    if (!(Rxy == Rxy) || !(Rz == Rz)) return fail("bins must not be NaN");
+    P.nb_count = (kind == HP_NPAIRS_XY_Z) ? P.Kr * P.Kp : P.Kr;
+
+    // ---- per-mock geometry
+    P.geom.resize((size_t)P.M);
+    int64_t p_off[2] = {0, 0}, g_off[2] = {0, 0}, cell_off = 0, key_off = 0;
+    for (int m = 0; m < P.M; m++) {
+        const hp_mock& hm = r->mocks[m];
+        MockGeom& G = P.geom[(size_t)m];
+        std::memset(&G, 0, sizeof(G));
+        const int64_t nm[2] = {hm.n1, P.same ? hm.n1 : hm.n2};
+        if (nm[0] < 0 || nm[1] < 0) return fail("negative point count");
+        double Lmax = 0.0;
+        int maxdiv = 1;
+        int64_t ncells = 1;
+        for (int d = 0; d < 3; d++) {
+            const double L = hm.period[d];
+            if (!(L > 0.0) || !(L < INFINITY)) return fail("period / box length must be positive and finite");
+            const double R = (d < 2) ? Rxy : Rz;
+            if (hm.pbc && R > L / 3.0 * (1.0 + 1e-12))
+                return fail("search length exceeds period/3 (the reference's mesh limit)");
+            double b = hm.cell[d] > 0.0 ? hm.cell[d] : 0.5 * R;
+            if (!(b > 0.0)) b = L;  // zero search radius
+            double nd = floor(L / b);
+            if (nd < 1.0) nd = 1.0;
+            if (nd > 1024.0) nd = 1024.0;
+            G.ndiv[d] = (int32_t)nd;
+            G.cell[d] = L / (double)G.ndiv[d];
+            G.period[d] = L;
+            Lmax = std::max(Lmax, L);
+            maxdiv = std::max(maxdiv, G.ndiv[d]);
+            ncells *= G.ndiv[d];
+        }
+        G.pbc = hm.pbc ? 1 : 0;
+        int mb = 0;
+        while ((1 << mb) < maxdiv) mb++;
+        G.mbits = mb;
+        const float eps = (float)(1e-5 * Lmax);
+        G.eps = eps;
+        G.search[0] = (float)(Rxy * (1.0 + 1e-6)) + eps;
+        G.search[1] = G.search[0];
+        G.search[2] = (float)(Rz * (1.0 + 1e-6)) + eps;
+        G.r2xy = G.search[0] * G.search[0] * (1.0f + 1e-6f);
+        G.shape = shape;
+        for (int s = 0; s < 2; s++) {
+            G.p_off[s] = p_off[s];
+            G.n[s] = nm[s];
+            G.group_off[s] = g_off[s];
+            G.ngroups[s] = (nm[s] + kGroup - 1) / kGroup;
+            P.max_groups[s] = std::max(P.max_groups[s], G.ngroups[s]);
+            p_off[s] += nm[s];
+            g_off[s] += G.ngroups[s];
+        }
+        G.cell_off = cell_off;
+        G.key_off = key_off;
+        cell_off += ncells;
+        key_off += (int64_t)1 << (3 * mb);
+        if (nm[1] > 0 && (double)nm[1] * 27.0 >= 4.0e9) return fail("too many points in one mock for 32-bit histograms");
+    }
+    P.n[0] = p_off[0];
+    P.n[1] = p_off[1];
+    if (P.n[0] >= (int64_t)0x7fffffff || P.n[1] >= (int64_t)0x7fffffff) return fail("too many points (>= 2^31)");
+    P.ngroups[0] = g_off[0];
+    P.ngroups[1] = g_off[1];
+    P.total_cells = cell_off;
+    P.total_keys = key_off;
+    if (cell_off >= (int64_t)0xffffffff || key_off > (int64_t)0xffffffff) return fail("mesh too large for 32-bit keys");
+    P.bits_row = bits_for((uint64_t)cell_off);
+    P.bits_morton = bits_for((uint64_t)key_off);
+
+    // ---- workspace layout
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = align_up(off, 256);
+        off = o + bytes;
+        return o;
+    };
+    P.o_geom = take(sizeof(MockGeom) * (size_t)P.M);
+    P.o_er = take(P.tr.e.size() * 8);
+    P.o_ep = take(P.tp.e.size() * 8);
+    P.o_tr = take(P.tr.t.size() * sizeof(TabEntry));
+    P.o_tp = take(P.tp.t.size() * sizeof(TabEntry));
+    const size_t consts_end = off;
+    P.o_status = take(16);
+    P.o_gm[0] = take((size_t)P.ngroups[0] * 4 + 4);
+    if (P.rows2) P.o_gm[1] = take((size_t)(P.same ? P.ngroups[0] : P.ngroups[1]) * 4 + 4);
+    // sorted sides: pass A lanes = (0, Morton), stream = (s2, RowMajor);
+    // pass B lanes = (s2, Morton), stream = (0, RowMajor)
+    const int s2 = P.same ? 0 : 1;
+    P.sorted[0][kMorton].needed = true;
+    P.sorted[s2][kRowMajor].needed = true;
+    if (P.rows2) {
+        P.sorted[s2][kMorton].needed = true;
+        P.sorted[0][kRowMajor].needed = true;
+    }
+    int64_t nmax = 1;
+    for (int s = 0; s < 2; s++) {
+        for (int o = 0; o < 2; o++) {
+            Side& S = P.sorted[s][o];
+            if (!S.needed) continue;
+            const size_t n = (size_t)std::max<int64_t>(P.n[s], 1);
+            nmax = std::max<int64_t>(nmax, (int64_t)n);
+            S.x = take(n * 8);
+            S.y = take(n * 8);
+            S.z = take(n * 8);
+            if (P.marked) S.w = take(n * 8 * (size_t)P.nw);
+            S.perm = take(n * 4);
+            if (o == kRowMajor) S.cs = take((size_t)(P.total_cells + 1) * 4);
+        }
+    }
+    const int64_t ntiles = (nmax + kSortTile - 1) / kSortTile;
+    P.o_keys[0] = take((size_t)nmax * 4);
+    P.o_keys[1] = take((size_t)nmax * 4);
+    P.o_vals[0] = take((size_t)nmax * 4);
+    P.o_vals[1] = take((size_t)nmax * 4);
+    P.o_tiles = take((size_t)ntiles * kRadixDigits * 4);
+    P.o_digits = take((size_t)kRadixDigits * 4);
+    const int64_t gA = std::max<int64_t>(P.ngroups[0], 1);
+    if (P.counts) P.o_pcount = take((size_t)gA * (size_t)P.nb_count * 8);
+    if (P.sums) P.o_psum = take((size_t)gA * (size_t)P.Kr * 8);
+    P.o_pevals = take((size_t)gA * 8);
+    const int64_t red = reduce_scratch_elems(std::max<int64_t>(P.max_groups[0], 1), P.M,
+                                             std::max(P.nb_count, P.Kr));
+    P.o_red = take((size_t)red * 8);
+    P.total = align_up(off, 256);
+
+    // ---- constant blob (geometry, edges, tables) that goes to the workspace
+    P.tr.spec.edges_off = (int64_t)P.o_er;
+    P.tr.spec.table_off = (int64_t)P.o_tr;
+    P.tp.spec.edges_off = (int64_t)P.o_ep;
+    P.tp.spec.table_off = (int64_t)P.o_tp;
+    P.consts.assign(consts_end, 0);
+    std::memcpy(P.consts.data() + P.o_geom, P.geom.data(), sizeof(MockGeom) * (size_t)P.M);
+    std::memcpy(P.consts.data() + P.o_er, P.tr.e.data(), P.tr.e.size() * 8);
+    std::memcpy(P.consts.data() + P.o_ep, P.tp.e.data(), P.tp.e.size() * 8);
+    std::memcpy(P.consts.data() + P.o_tr, P.tr.t.data(), P.tr.t.size() * sizeof(TabEntry));
+    std::memcpy(P.consts.data() + P.o_tp, P.tp.t.data(), P.tp.t.size() * sizeof(TabEntry));
+    return 0;
+}
+
+static SortedSide side_ptrs(char* ws, const Side& S) {
+    SortedSide o;
+    o.x = reinterpret_cast<double*>(ws + S.x);
+    o.y = reinterpret_cast<double*>(ws + S.y);
+    o.z = reinterpret_cast<double*>(ws + S.z);
+    o.w = S.w ? reinterpret_cast<double*>(ws + S.w) : nullptr;
+    o.perm = reinterpret_cast<uint32_t*>(ws + S.perm);
+    o.cell_start = S.cs ? reinterpret_cast<uint32_t*>(ws + S.cs) : nullptr;
+    return o;
+}
+
+static int run(const hp_request* r, int kind) {
+    Plan P;
+    if (make_plan(r, kind, P)) return -1;
+    if (!r->workspace) return fail("workspace must not be null");
+    if (r->workspace_bytes < P.total) return fail("workspace too small");
+    if (P.n[0] > 0 && (!r->x1 || !r->y1 || !r->z1)) return fail("sample1 coordinates must not be null");
+    if (!P.same && P.n[1] > 0 && (!r->x2 || !r->y2 || !r->z2))
+        return fail("sample2 coordinates must not be null");
+    if (P.marked && ((P.n[0] > 0 && !r->w1) || (!P.same && P.n[1] > 0 && !r->w2)))
+        return fail("weights must not be null");
+    if (P.counts && !r->counts) return fail("counts output must not be null");
+    if (P.sums && !r->sums) return fail("sums output must not be null");
+    if (P.rows1 && !r->rows1) return fail("rows1 output must not be null");
+    if (P.rows2 && !r->rows2) return fail("rows2 output must not be null");
+    hipStream_t s = static_cast<hipStream_t>(r->stream);
+    char* ws = static_cast<char*>(r->workspace);
+    const MockGeom* dgeom = reinterpret_cast<const MockGeom*>(ws + P.o_geom);
+
+    if (launch_store_bytes(ws, P.consts.data(), P.consts.size(), s)) return fail("constant upload failed");
+    int32_t* status = r->status ? r->status : reinterpret_cast<int32_t*>(ws + P.o_status);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(status), 0u, 1, s);
+
+    const double* X[2] = {r->x1, P.same ? r->x1 : r->x2};
+    const double* Y[2] = {r->y1, P.same ? r->y1 : r->y2};
+    const double* Z[2] = {r->z1, P.same ? r->z1 : r->z2};
+    const double* W[2] = {r->w1, P.same ? r->w1 : r->w2};
+
+    SortBufs sb;
+    sb.keys[0] = reinterpret_cast<uint32_t*>(ws + P.o_keys[0]);
+    sb.keys[1] = reinterpret_cast<uint32_t*>(ws + P.o_keys[1]);
+    sb.vals[0] = reinterpret_cast<uint32_t*>(ws + P.o_vals[0]);
+    sb.vals[1] = reinterpret_cast<uint32_t*>(ws + P.o_vals[1]);
+    sb.tile_counts = reinterpret_cast<uint32_t*>(ws + P.o_tiles);
+    sb.digit_totals = reinterpret_cast<uint32_t*>(ws + P.o_digits);
+    for (int sd = 0; sd < 2; sd++) {
+        for (int o = 0; o < 2; o++) {
+            const Side& S = P.sorted[sd][o];
+            if (!S.needed) continue;
+            // with SAME_SAMPLES only side 0 exists; the geometry rows of side 0 describe it
+            if (build_sorted_side(dgeom, P.M, sd, (Order)o, X[sd], Y[sd], Z[sd], P.marked ? W[sd] : nullptr, P.nw,
+                                  P.n[sd], P.total_cells, o == kRowMajor ? P.bits_row : P.bits_morton, sb,
+                                  side_ptrs(ws, S), status, s))
+                return fail("cell build launch failed");
+        }
+    }
+
+    const int s2 = P.same ? 0 : 1;
+    int32_t* gm0 = reinterpret_cast<int32_t*>(ws + P.o_gm[0]);
+    if (launch_group_mock(dgeom, P.M, 0, gm0, P.ngroups[0], s)) return fail("group table launch failed");
+
+    PairLaunch L;
+    std::memset(&L, 0, sizeof(L));
+    L.geom = dgeom;
+    L.group_mock = gm0;
+    L.ngroups = P.ngroups[0];
+    L.lane = side_ptrs(ws, P.sorted[0][kMorton]);
+    L.strm = side_ptrs(ws, P.sorted[s2][kRowMajor]);
+    L.nw = P.nw;
+    L.br = P.tr.spec;
+    L.bp = P.tp.spec;
+    L.ws = ws;
+    L.wfunc = r->wfunc;
+    L.pcount = P.counts ? reinterpret_cast<uint64_t*>(ws + P.o_pcount) : nullptr;
+    L.psum = P.sums ? reinterpret_cast<double*>(ws + P.o_psum) : nullptr;
+    L.pevals = reinterpret_cast<uint64_t*>(ws + P.o_pevals);
+    switch (kind) {
+        case HP_NPAIRS_3D: L.mode = kModeCount3D; break;
+        case HP_NPAIRS_PROJECTED: L.mode = kModeProj; break;
+        case HP_NPAIRS_XY_Z: L.mode = kModeXYZ; break;
+        default:
+            L.mode = (r->wfunc == 1 && P.nw == 1 && (P.rows1 || P.rows2)) ? kModeJac : kModeWSum;
+            L.also_count = P.counts ? 1 : 0;
+            L.rows = P.rows1 ? r->rows1 : nullptr;
+            break;
+    }
+    int rc = 0;
+    if (!(P.marked && !P.sums && !P.counts && !P.rows1)) {
+        rc = launch_pairs(L, s);
+        if (rc == -2) return fail("histogram does not fit in LDS (too many bins)");
+        if (rc) return fail("pair kernel launch failed");
+    }
+    if (P.rows2) {
+        // pass B: lanes are sample 2, the stream is sample 1 (same arithmetic,
+        // (x1 - x2shift) - x2, with the shift applied to the sample-1 point)
+        int32_t* gm1 = reinterpret_cast<int32_t*>(ws + P.o_gm[1]);
+        if (launch_group_mock(dgeom, P.M, s2, gm1, P.ngroups[s2], s)) return fail("group table launch failed");
+        PairLaunch B = L;
+        B.mode = kModeJac;
+        B.swap = 1;
+        B.also_count = 0;
+        B.group_mock = gm1;
+        B.ngroups = P.ngroups[s2];
+        B.lane = side_ptrs(ws, P.sorted[s2][kMorton]);
+        B.strm = side_ptrs(ws, P.sorted[0][kRowMajor]);
+        B.rows = r->rows2;
+        B.pcount = nullptr;
+        B.psum = nullptr;
+        B.pevals = nullptr;
+        // with SAME_SAMPLES the side-1 rows of the geometry equal the side-0 rows
+        if (launch_pairs(B, s)) return fail("pair kernel (swapped) launch failed");
+    }
+    uint64_t* red = reinterpret_cast<uint64_t*>(ws + P.o_red);
+    if (P.counts) {
+        if (launch_reduce_counts_s(L.pcount, P.Kr, kind == HP_NPAIRS_XY_Z ? P.Kp : 0, dgeom, P.M, 0,
+                                   P.max_groups[0], red, r->counts, s))
+            return fail("reduction launch failed");
+    }
+    if (P.sums) {
+        if (launch_reduce_sums_s(L.psum, P.Kr, dgeom, P.M, 0, P.max_groups[0], reinterpret_cast<double*>(red),
+                                 r->sums, s))
+            return fail("reduction launch failed");
+    }
+    if (r->evals) {
+        if (launch_reduce_evals_s(L.pevals, dgeom, P.M, 0, P.max_groups[0], red, r->evals, s))
+            return fail("reduction launch failed");
+    }
+    if (hipGetLastError() != hipSuccess) return fail("HIP launch error");
+    g_err.clear();
+    return 0;
+}
+
+extern "C" {
+
+const char* hp_last_error(void) { return g_err.c_str(); }
+
+int hp_abi_version(void) { return HP_ABI_VERSION; }
+
+size_t hp_workspace_size(const hp_request* req) {
+    Plan P;
+    if (!req) return 0;
+    if (make_plan(req, req->kind, P)) return 0;
+    return P.total;
+}
+
+int hp_npairs_3d(const hp_request* req) { return run(req, HP_NPAIRS_3D); }
+int hp_npairs_projected(const hp_request* req) { return run(req, HP_NPAIRS_PROJECTED); }
+int hp_npairs_xy_z(const hp_request* req) { return run(req, HP_NPAIRS_XY_Z); }
+int hp_marked_npairs_3d(const hp_request* req) { return run(req, HP_MARKED_NPAIRS_3D); }
+
+int hp_marked_backward(const double* rows, int32_t nr, const double* gsuffix, const int64_t* offsets,
+                       int32_t nmocks, double* grad, void* stream) {
+    if (!rows || !gsuffix || !offsets || !grad) return fail("null pointer");
+    if (nr < 1 || nmocks < 1) return fail("bad sizes");
+    if (launch_rows_dot(rows, nr, gsuffix, offsets, nmocks, grad, static_cast<hipStream_t>(stream)))
+        return fail("backward launch failed");
+    g_err.clear();
+    return 0;
+}
+
+// Host-only diagnostics: the bucketed bin lookup the kernels use, evaluated on
+// the CPU for a set of squared distances (tests pin it against searchsorted).
+int hp_debug_bin_lookup(const double* r2, int32_t K, const double* d, int64_t n, int32_t* out, int32_t* info) {
+    if (!r2 || !d || !out || K < 1) return fail("bad arguments");
+    std::vector<double> e = suffix_min(r2, K);
+    Table T = make_table(e, K);
+    for (int64_t i = 0; i < n; i++) out[i] = lookup_host(T, d[i]);
+    if (info) {
+        info[0] = T.spec.shift;
+        info[1] = T.spec.base;
+        info[2] = T.spec.nb;
+        info[3] = T.spec.multi;
+    }
+    return 0;
+}
+
+}  // extern "C"

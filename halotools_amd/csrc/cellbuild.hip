@@ -1,0 +1,280 @@
+// cellbuild.hip -- GPU cell-list build (replaces RectangularMesh,
+// PC/rectangular_mesh.py:181-225, whose argsort + searchsorted bucket the
+// points of each sample into a uniform grid).
+//
+// The build is a stable LSD counting sort of 32-bit cell keys (8-bit digits,
+// one wavefront per 2048-key tile), so the order of points inside a cell is
+// their input order and every downstream floating-point sum is run-to-run
+// deterministic.  Two key flavours exist:
+//   row-major  (ix*ny + iy)*nz + iz  -- stream side; the cells of one (ix, iy)
+//              column are contiguous, so a column's z-range is one segment;
+//   Morton     interleaved bits     -- lane side; 64 consecutive points form a
+//              spatially compact group.
+// A key of mock m is offset by that mock's first cell (row-major) or first
+// Morton code, so one sort handles a whole batch of realisations.
+#include <cstring>
+
+#include "hp_internal.h"
+
+namespace hp {
+
+// ----------------------------------------------------------------- utilities
+struct ByteChunk {
+    uint32_t* dst;
+    uint32_t nwords;
+    uint32_t words[508];  // 2032 bytes of payload per launch
+};
+
+__global__ void k_store_words(const ByteChunk c) {
+    for (uint32_t i = threadIdx.x; i < c.nwords; i += blockDim.x) c.dst[i] = c.words[i];
+}
+
+int launch_store_bytes(void* dst, const void* src, size_t nbytes, hipStream_t s) {
+    // Carries a host blob into device memory through kernel arguments: no
+    // memcpy API, so the upload is legal inside HIP-graph capture.
+    const size_t nwords_total = (nbytes + 3) / 4;
+    const uint8_t* bytes = static_cast<const uint8_t*>(src);
+    for (size_t w0 = 0; w0 < nwords_total; w0 += 508) {
+        ByteChunk c;
+        const size_t nw = nwords_total - w0 < 508 ? nwords_total - w0 : 508;
+        c.dst = static_cast<uint32_t*>(dst) + w0;
+        c.nwords = static_cast<uint32_t>(nw);
+        const size_t b0 = w0 * 4;
+        const size_t nb = (b0 + nw * 4 <= nbytes) ? nw * 4 : nbytes - b0;
+        std::memset(c.words, 0, sizeof(c.words));
+        std::memcpy(c.words, bytes + b0, nb);
+        hipLaunchKernelGGL(k_store_words, dim3(1), dim3(256), 0, s, c);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s) {
+    if (n <= 0) return 0;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+__global__ void k_group_mock(const MockGeom* __restrict__ g, int nmocks, int side, int32_t* __restrict__ gm,
+                             int64_t ngroups) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= ngroups) return;
+    int lo = 0, hi = nmocks - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (g[mid].group_off[side] <= q) lo = mid; else hi = mid - 1;
+    }
+    gm[q] = lo;
+}
+
+int launch_group_mock(const MockGeom* g, int nmocks, int side, int32_t* gm, int64_t ngroups, hipStream_t s) {
+    if (ngroups <= 0) return 0;
+    hipLaunchKernelGGL(k_group_mock, dim3((unsigned)((ngroups + 255) / 256)), dim3(256), 0, s, g, nmocks, side,
+                       gm, ngroups);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ----------------------------------------------------------------- cell keys
+// Cell index of a coordinate: floor(v / cell), clamped into [0, n-1] so that
+// v == period lands in the last cell (as PC/rectangular_mesh.py:19-22 does).
+// The pair kernels plan their windows with a safety margin, so the exact
+// rounding of this division does not affect which pairs are counted.
+__device__ __forceinline__ int cell_index(double v, double cell, int n, bool& bad, double period, int pbc) {
+    if (!(v == v) || v > 1.0e300 || v < -1.0e300) { bad = true; return 0; }
+    if (pbc && (v < 0.0 || v > period)) bad = true;
+    const double f = floor(v / cell);
+    if (!(f >= 0.0)) return 0;
+    if (f >= (double)n) return n - 1;
+    return (int)f;
+}
+
+__global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, int order,
+                       const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+                       int64_t n, uint32_t* __restrict__ keys, int32_t* __restrict__ status) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int64_t m = mock_of_point(geom, nmocks, side, p);
+    const MockGeom& G = geom[m];
+    bool bad = false;
+    const int ix = cell_index(x[p], G.cell[0], G.ndiv[0], bad, G.period[0], G.pbc);
+    const int iy = cell_index(y[p], G.cell[1], G.ndiv[1], bad, G.period[1], G.pbc);
+    const int iz = cell_index(z[p], G.cell[2], G.ndiv[2], bad, G.period[2], G.pbc);
+    uint32_t key;
+    if (order == kRowMajor) {
+        key = (uint32_t)(G.cell_off + ((int64_t)ix * G.ndiv[1] + iy) * G.ndiv[2] + iz);
+    } else {
+        const uint32_t code = (part1by2((uint32_t)ix) << 2) | (part1by2((uint32_t)iy) << 1) | part1by2((uint32_t)iz);
+        key = (uint32_t)(G.key_off + code);
+    }
+    keys[p] = key;
+    if (bad && status) atomicOr(status, 1);
+}
+
+// ----------------------------------------------------- LSD counting sort
+__global__ void __launch_bounds__(256) k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                    uint32_t* __restrict__ tile_counts, int ntiles) {
+    __shared__ uint32_t h[kRadixDigits];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kSortTile;
+    for (int i = threadIdx.x; i < kSortTile; i += 256) {
+        const int64_t idx = base + i;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (kRadixDigits - 1)], 1u);
+    }
+    __syncthreads();
+    tile_counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of each digit's row of per-tile counts; row totals out.
+__global__ void __launch_bounds__(256) k_radix_scan(uint32_t* __restrict__ tile_counts, int ntiles,
+                                                    uint32_t* __restrict__ digit_totals) {
+    __shared__ uint32_t part[256];
+    uint32_t* row = tile_counts + (int64_t)blockIdx.x * ntiles;
+    const int per = (ntiles + 255) / 256;
+    const int b0 = threadIdx.x * per;
+    const int b1 = min(b0 + per, ntiles);
+    uint32_t sum = 0;
+    for (int i = b0; i < b1; i++) sum += row[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan of 256 partial sums
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0;
+    for (int i = b0; i < b1; i++) {
+        const uint32_t c = row[i];
+        row[i] = run;
+        run += c;
+    }
+    if (threadIdx.x == 255) digit_totals[blockIdx.x] = part[255];
+}
+
+// One wavefront per tile.  Keys are placed in tile order, and inside a tile
+// in index order: a lane's rank among the lanes holding the same digit comes
+// from eight ballots, so the sort is stable without any atomics.
+__global__ void __launch_bounds__(64) k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                      uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                      int64_t n, int shift, const uint32_t* __restrict__ tile_counts,
+                                                      const uint32_t* __restrict__ digit_totals, int ntiles) {
+    __shared__ uint32_t run[kRadixDigits];
+    const int lane = threadIdx.x;
+    const int tile = blockIdx.x;
+    // digit bases: exclusive scan of the 256 totals, 4 digits per lane
+    uint32_t t[4];
+    uint32_t local = 0;
+    for (int q = 0; q < 4; q++) { t[q] = digit_totals[lane * 4 + q]; local += t[q]; }
+    uint32_t incl = local;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    uint32_t b = incl - local;
+    for (int q = 0; q < 4; q++) {
+        const int d = lane * 4 + q;
+        run[d] = b + tile_counts[(int64_t)d * ntiles + tile];
+        b += t[q];
+    }
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int64_t base = (int64_t)tile * kSortTile;
+    for (int r = 0; r < kSortTile / kWave; r++) {
+        const int64_t idx = base + r * kWave + lane;
+        if (base + r * kWave >= n) break;
+        const bool valid = idx < n;
+        const uint32_t key = valid ? kin[idx] : 0u;
+        const uint32_t val = valid ? (vin ? vin[idx] : (uint32_t)idx) : 0u;
+        const uint32_t d = (key >> shift) & (kRadixDigits - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < kRadixBits; bit++) {
+            const bool on = (d >> bit) & 1u;
+            const uint64_t bm = __ballot(on);
+            peers &= on ? bm : ~bm;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        const uint32_t pos = run[d] + rank;
+        if (valid) {
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) run[d] = pos + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Sorted copies of coordinates (and weight rows) in the new order.
+__global__ void k_gather(const uint32_t* __restrict__ perm, int64_t n, const double* __restrict__ x,
+                         const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ w,
+                         int nw, double* __restrict__ ox, double* __restrict__ oy, double* __restrict__ oz,
+                         double* __restrict__ ow) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int64_t p = perm[q];
+    ox[q] = x[p];
+    oy[q] = y[p];
+    oz[q] = z[p];
+    if (w) {
+        for (int c = 0; c < nw; c++) ow[q * nw + c] = w[p * nw + c];
+    }
+}
+
+// cell_start[c] = first sorted position whose key >= c, for c in [0, total].
+__global__ void k_cell_start(const uint32_t* __restrict__ keys, int64_t n, int64_t total_cells,
+                             uint32_t* __restrict__ cell_start) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q > n) return;
+    const int64_t kcur = (q < n) ? (int64_t)keys[q] : total_cells;
+    const int64_t kprev = (q > 0) ? (int64_t)keys[q - 1] : -1;
+    for (int64_t c = kprev + 1; c <= kcur && c <= total_cells; c++) cell_start[c] = (uint32_t)q;
+}
+
+static inline int check(const char*) { return hipGetLastError() == hipSuccess ? 0 : -1; }
+
+int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, const double* x, const double* y,
+                      const double* z, const double* w, int nw, int64_t n, int64_t total_cells, int key_bits,
+                      SortBufs sb, SortedSide out, int32_t* status, hipStream_t s) {
+    if (n <= 0) {
+        if (order == kRowMajor) return launch_fill_u32(out.cell_start, 0u, total_cells + 1, s);
+        return 0;
+    }
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_keys, dim3(nb), dim3(256), 0, s, geom, nmocks, side, (int)order, x, y, z, n, sb.keys[0],
+                       status);
+    const int ntiles = (int)((n + kSortTile - 1) / kSortTile);
+    const int passes = key_bits <= 0 ? 1 : (key_bits + kRadixBits - 1) / kRadixBits;
+    int cur = 0;
+    for (int pass = 0; pass < passes; pass++) {
+        const int shift = pass * kRadixBits;
+        hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(256), 0, s, sb.keys[cur], n, shift, sb.tile_counts,
+                           ntiles);
+        hipLaunchKernelGGL(k_radix_scan, dim3(kRadixDigits), dim3(256), 0, s, sb.tile_counts, ntiles,
+                           sb.digit_totals);
+        // the last pass writes the permutation straight into out.perm
+        uint32_t* vdst = (pass == passes - 1) ? out.perm : sb.vals[cur ^ 1];
+        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(64), 0, s, sb.keys[cur],
+                           pass == 0 ? (const uint32_t*)nullptr : sb.vals[cur], sb.keys[cur ^ 1], vdst, n,
+                           shift, sb.tile_counts, sb.digit_totals, ntiles);
+        cur ^= 1;
+    }
+    if (check("sort")) return -1;
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, s, out.perm, n, x, y, z, w, nw, out.x, out.y, out.z,
+                       out.w);
+    if (order == kRowMajor) {
+        hipLaunchKernelGGL(k_cell_start, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, sb.keys[cur], n,
+                           total_cells, out.cell_start);
+    }
+    return check("gather");
+}
+
+}  // namespace hp

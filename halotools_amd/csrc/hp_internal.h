@@ -1,0 +1,141 @@
+// hp_internal.h -- device-side layout shared by the HIP translation units of
+// libhalopair.so.  Not part of the public ABI (see include/halopair.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/halopair.h"
+
+namespace hp {
+
+constexpr int kWave = 64;            // CDNA wavefront width
+constexpr int kGroup = 64;           // lane-side points per wave work item
+constexpr int kTableMax = 512;       // bucket table entries per axis
+constexpr int kRadixBits = 8;        // digit width of the LSD counting sort
+constexpr int kRadixDigits = 1 << kRadixBits;
+constexpr int kSortTile = 2048;      // keys per counting-sort tile (one wave)
+
+// Per-mock geometry, resident in the workspace.  "side" 0 is sample 1, side 1
+// is sample 2.  Cells of the GPU mesh are shared by both sides.
+struct alignas(64) MockGeom {
+    double period[3];     // box lengths (PBC period, or enclosing box)
+    double cell[3];       // fine-cell edge per dimension
+    int32_t ndiv[3];      // cells per dimension
+    int32_t pbc;
+    int32_t mbits;        // bits per dimension of the Morton code
+    int32_t pad;
+    int64_t p_off[2];     // first point of this mock on each side
+    int64_t n[2];         // points of this mock on each side
+    int64_t cell_off;     // first global cell id of this mock
+    int64_t key_off;      // first global Morton key of this mock
+    int64_t group_off[2]; // first lane group of this mock, per lane side
+    int64_t ngroups[2];
+    float search[3];      // search half-widths (window), with safety margin
+    float eps;            // absolute safety margin of the f32 window planning
+    float r2xy;           // squared planar search radius (with margin)
+    int32_t shape;        // 0: sphere (3-D), 1: cylinder (rp, pi)
+};
+
+// Bucketed bin lookup for an ascending edge array e[0..K-1], e[K] = +inf.
+// bucket(d) = min(sat(hi32(d) - (base << shift)) >> shift, nb-1); with at most
+// one edge inside any bucket (multi == 0) the bin start is
+// table[b].k + (table[b].edge < d); otherwise a short scan finishes the job.
+struct BinSpec {
+    int32_t nbins;   // K
+    int32_t shift;
+    int32_t base;    // >= 0
+    uint32_t basesh; // base << shift
+    int32_t nb;      // table entries
+    int32_t multi;   // 1 if some bucket holds more than one edge
+    int64_t edges_off;  // byte offset of e[0..K+1] in the workspace
+    int64_t table_off;  // byte offset of TabEntry table[nb]
+};
+
+// One bucket of the table: k = number of edges strictly below the bucket's
+// smallest value, edge = e[k] (the only edge that can lie inside the bucket).
+struct alignas(16) TabEntry {
+    double edge;
+    uint32_t k;
+    uint32_t pad;
+};
+
+// Counting-sort scratch of one sort
+struct SortBufs {
+    uint32_t* keys[2];
+    uint32_t* vals[2];
+    uint32_t* tile_counts;   // [digits][ntiles]
+    uint32_t* digit_totals;  // [digits]
+};
+
+enum Order : int { kMorton = 0, kRowMajor = 1 };
+
+// Sorted copy of one side in one order.
+struct SortedSide {
+    double* x;
+    double* y;
+    double* z;
+    double* w;          // (n, nw) row-major, or nullptr
+    uint32_t* perm;     // sorted position -> original index
+    uint32_t* cell_start;  // row-major order only: [total_cells + 1]
+};
+
+// ------------------------------------------------------------------ launchers
+// cellbuild.hip
+int launch_store_bytes(void* dst, const void* src, size_t nbytes, hipStream_t s);
+int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+int launch_group_mock(const MockGeom* g, int nmocks, int side, int32_t* group_mock,
+                      int64_t ngroups_total, hipStream_t s);
+int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order,
+                      const double* x, const double* y, const double* z, const double* w,
+                      int nw, int64_t n, int64_t total_cells, int key_bits,
+                      SortBufs sb, SortedSide out, int32_t* status, hipStream_t s);
+
+// pairs.hip
+struct PairLaunch {
+    int mode;              // kModeCount3D ...
+    int swap;              // 1: lanes are sample 2
+    int also_count;        // marked: also accumulate unweighted counts
+    int wfunc;             // marked weight function id
+    const MockGeom* geom;
+    const int32_t* group_mock;
+    int64_t ngroups;
+    SortedSide lane;       // Morton-sorted lane side
+    SortedSide strm;       // row-major sorted stream side
+    int nw;
+    BinSpec br;            // radial / rp bins
+    BinSpec bp;            // pi bins (xy_z) or pi_max (projected, nbins = 1)
+    const void* ws;        // workspace base (edges / tables)
+    uint64_t* pcount;      // [ngroups][nbins_count]
+    double* psum;          // [ngroups][nr]
+    double* rows;          // [n_lane_total][nr] (original order) or nullptr
+    uint64_t* pevals;      // [ngroups]
+};
+enum Mode : int { kModeCount3D = 0, kModeProj = 1, kModeXYZ = 2, kModeWSum = 3, kModeJac = 4 };
+int launch_pairs(const PairLaunch& L, hipStream_t s);
+
+// reduce.hip
+int launch_rows_dot(const double* rows, int nr, const double* gsuffix, const int64_t* offsets,
+                    int nmocks, double* grad, hipStream_t s);
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ int64_t mock_of_point(const MockGeom* __restrict__ g, int nmocks, int side,
+                                                 int64_t p) {
+    int lo = 0, hi = nmocks - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (g[mid].p_off[side] <= p) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t part1by2(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+}  // namespace hp

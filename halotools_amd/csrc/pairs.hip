@@ -1,0 +1,502 @@
+// pairs.hip -- cell-pair tile kernels (replace the j-loops of
+// PC/cpairs/npairs_3d_engine.pyx:103-182, PC/cpairs/npairs_projected_engine.pyx:103-190,
+// PC/cpairs/npairs_xy_z_engine.pyx:103-194 and
+// PC/marked_cpairs/marked_npairs_3d_engine.pyx:103-216).
+//
+// Work decomposition (MI355X-first, not the reference's cell loop):
+//   * one wavefront = one "group" of 64 lane points that are consecutive in the
+//     Morton-sorted copy of the lane side, hence spatially compact;
+//   * the wave reduces its points' bounding box, walks the (x, y) columns of
+//     fine cells within the search radius of that box, and for each column
+//     visits one contiguous z-run of the row-major sorted stream side (split
+//     at the periodic seam);
+//   * stream points are wave-uniform, so they arrive through scalar loads and
+//     sit in SGPRs; every lane evaluates its own point against them;
+//   * the bin start of each pair comes from a bucketed edge table in LDS and
+//     one exact FP64 comparison; histograms are per lane in LDS (bin-major,
+//     bank-conflict free) and are reduced once per group in a fixed order.
+//
+// Exactness: every pair is evaluated with the reference's arithmetic,
+// x1tmp = x1 - x2shift (x2shift in {-L, 0, +L}, chosen by which periodic image
+// of the stream cell is visited), dx = x1tmp - x2, dsq = dx*dx + dy*dy + dz*dz,
+// with FP contraction off, and binned with the same "<=" semantics.  A pair
+// within the search radius is visited exactly once with its minimum image;
+// other images of it lie at least L - rmax >= 2 rmax away and bin nowhere.
+#include "hp_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace hp {
+
+// ------------------------------------------------------------- marking functions
+// PC/marked_cpairs/marking_functions.pyx:14-217 (ids 1-17).
+__device__ __forceinline__ double wfunc_eval(int id, const double* a, const double* b) {
+    switch (id) {
+        case 1: return a[0] * b[0];
+        case 2: return a[0] + b[0];
+        case 3: return a[0] == b[0] ? a[1] * b[1] : 0.0;
+        case 4: return a[0] != b[0] ? a[1] * b[1] : 0.0;
+        case 5: return b[0] > a[0] ? a[1] * b[1] : 0.0;
+        case 6: return b[0] < a[0] ? a[1] * b[1] : 0.0;
+        case 7: return b[0] > (a[0] + a[1]) ? b[1] : 0.0;
+        case 8: return b[0] < (a[0] + a[1]) ? b[1] : 0.0;  // '+' as in the reference (:104-115)
+        case 9: return fabs(a[0] - b[0]) < a[1] ? b[1] : 0.0;
+        case 10: return fabs(a[0] - b[0]) > a[1] ? b[1] : 0.0;
+        case 11: return b[0] > a[0] * a[1] ? b[1] : 0.0;
+        case 12: return a[0] * b[0] * (a[1] * b[1] + a[2] * b[2] + a[3] * b[3]);
+        case 13: {
+            const double c = a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+            return a[0] * b[0] * (a[1] * b[1] + a[2] * b[2] + a[3] * b[3]) * c;
+        }
+        case 14: return a[0] * b[0] * (a[1] * b[1] + a[2] * b[2]);
+        case 15: {
+            const double c = a[1] * b[1] + a[2] * b[2];
+            return a[0] * b[0] * (a[1] * b[1] + a[2] * b[2]) * c;
+        }
+        case 16: {
+            if (a[4] != b[4]) return 0.0;
+            const double c = a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+            return a[0] * b[0] * (a[1] * b[1] + a[2] * b[2] + a[3] * b[3]) * c;
+        }
+        case 17: {
+            if (a[4] == b[4]) return 0.0;
+            const double c = a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+            return a[0] * b[0] * (a[1] * b[1] + a[2] * b[2] + a[3] * b[3]) * c;
+        }
+        default: return 0.0;
+    }
+}
+
+// ------------------------------------------------------------- helpers
+__device__ __forceinline__ float f32_down(double v) {
+    float f = (float)v;
+    if ((double)f > v) f = nextafterf(f, -__builtin_inff());
+    return f;
+}
+__device__ __forceinline__ float f32_up(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = nextafterf(f, __builtin_inff());
+    return f;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+// butterfly: every lane ends with bitwise the same, order-fixed sum
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ float unif(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// LDS copy of a bucket entry: the exact edge to compare with, and the byte
+// offset of the histogram row the pair lands in when it does not exceed it.
+struct alignas(16) LdsEntry {
+    double edge;
+    uint32_t addr;
+    uint32_t k;
+};
+
+// Byte offset of the first bin a value d falls in (bin k iff d <= e[k]):
+// bucket from the high word of d, one exact FP64 comparison with the only
+// edge that can lie inside the bucket (MULTI == 0), or a scan otherwise.
+template <int MULTI>
+__device__ __forceinline__ uint32_t bin_addr(double d, const LdsEntry* __restrict__ tab, uint32_t shift,
+                                             uint32_t basesh, uint32_t nbm1, uint32_t step,
+                                             const double* __restrict__ gedges) {
+    const uint32_t hi = (uint32_t)__double2hiint(d);
+    const uint32_t b = min(__builtin_elementwise_sub_sat(hi, basesh) >> shift, nbm1);
+    const LdsEntry e = tab[b];
+    if (!MULTI) return e.addr + (e.edge < d ? step : 0u);
+    uint32_t k = e.k;
+    while (gedges[k] < d) ++k;
+    return k * step;
+}
+
+__device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb) {
+    // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous run of (Morton-ordered) groups so its L2 sees one region.
+    const int64_t q = nb / 8, r = nb % 8;
+    const int64_t x = bid % 8, k = bid / 8;
+    return x * q + (x < r ? x : r) + k;
+}
+
+struct PairArgs {
+    PairLaunch L;
+    int wave_bytes;     // histogram bytes per wave
+    int cnt_off;        // marked + counts: byte offset of the count rows
+    int shared_count;   // 1: one histogram per wave instead of per lane
+    int smem;           // dynamic LDS bytes
+};
+
+template <int MODE, int SWAP, int ALSO, int MULTI, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
+    constexpr bool F64 = (MODE == kModeWSum || MODE == kModeJac);
+    __shared__ LdsEntry s_tr[kTableMax];
+    __shared__ LdsEntry s_tp[MODE == kModeXYZ ? kTableMax : 1];
+    extern __shared__ __align__(16) unsigned char smem[];
+    const PairLaunch& L = A.L;
+    const int Kr = L.br.nbins, Kp = L.bp.nbins;
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+
+    // histogram geometry: rows of one bin; (Kr+1) rows, the last one a trash
+    // row for pairs beyond the largest edge (no branch in the pair loop)
+    const bool shared = (MODE == kModeXYZ) || (!F64 && A.shared_count);
+    const uint32_t step_r = (MODE == kModeXYZ) ? (uint32_t)(Kp + 1) * 4u : (shared ? 4u : (F64 ? 512u : 256u));
+    const uint32_t lane_off = shared ? 0u : (uint32_t)lane * (F64 ? 8u : 4u);
+    {
+        const char* ws = static_cast<const char*>(L.ws);
+        const TabEntry* gtr = reinterpret_cast<const TabEntry*>(ws + L.br.table_off);
+        for (int i = threadIdx.x; i < L.br.nb; i += blockDim.x) {
+            const TabEntry e = gtr[i];
+            s_tr[i].edge = e.edge;
+            s_tr[i].addr = e.k * step_r;
+            s_tr[i].k = e.k;
+        }
+        if (MODE == kModeXYZ) {
+            const TabEntry* gtp = reinterpret_cast<const TabEntry*>(ws + L.bp.table_off);
+            for (int i = threadIdx.x; i < L.bp.nb; i += blockDim.x) {
+                const TabEntry e = gtp[i];
+                s_tp[i].edge = e.edge;
+                s_tp[i].addr = e.k * 4u;
+                s_tp[i].k = e.k;
+            }
+        }
+        uint32_t* h = reinterpret_cast<uint32_t*>(smem);
+        const int nwords = (WAVES * A.wave_bytes) / 4;
+        for (int i = threadIdx.x; i < nwords; i += blockDim.x) h[i] = 0u;
+    }
+    __syncthreads();
+
+    const int64_t g = xcd_remap(blockIdx.x, gridDim.x) * WAVES + wid;
+    if (g >= L.ngroups) return;  // no block-level synchronisation below
+
+    unsigned char* const wsm = smem + wid * A.wave_bytes;  // this wave's histogram rows
+    unsigned char* const hl = wsm + lane_off;               // ... at this lane's column
+    const double* gedges_r = reinterpret_cast<const double*>(static_cast<const char*>(L.ws) + L.br.edges_off);
+    const double* gedges_p = reinterpret_cast<const double*>(static_cast<const char*>(L.ws) + L.bp.edges_off);
+
+    const int m = uni(L.group_mock[g]);
+    const MockGeom& G = L.geom[m];
+    const int side = SWAP;
+    const int64_t gl = g - G.group_off[side];
+    const int64_t p0 = G.p_off[side] + gl * kGroup;
+    const int64_t left = G.n[side] - gl * kGroup;
+    const int cnt = uni((int)(left < kGroup ? left : kGroup));
+    const bool act = lane < cnt;
+    const int64_t p = p0 + (act ? lane : 0);
+    // inactive lanes carry NaN: every comparison fails, so they land in the trash row
+    const double qnan = __builtin_nan("");
+    const double px = act ? L.lane.x[p] : qnan;
+    const double py = act ? L.lane.y[p] : qnan;
+    const double pz = act ? L.lane.z[p] : qnan;
+
+    const double Lx = G.period[0], Ly = G.period[1], Lz = G.period[2];
+    const int nx = G.ndiv[0], ny = G.ndiv[1], nz = G.ndiv[2];
+    const int pbc = G.pbc;
+    const int64_t cell_off = G.cell_off;
+
+    // ---- bounding box of the group (f32, rounded outwards)
+    const float INF = __builtin_inff();
+    const float bxlo = unif(wave_min(act ? f32_down(px) : INF));
+    const float bylo = unif(wave_min(act ? f32_down(py) : INF));
+    const float bzlo = unif(wave_min(act ? f32_down(pz) : INF));
+    const float bxhi = unif(wave_max(act ? f32_up(px) : -INF));
+    const float byhi = unif(wave_max(act ? f32_up(py) : -INF));
+    const float bzhi = unif(wave_max(act ? f32_up(pz) : -INF));
+
+    const float eps = G.eps;
+    const float Rx = G.search[0], Ry = G.search[1], Rz = G.search[2];
+    const float cxf = (float)G.cell[0], cyf = (float)G.cell[1], czf = (float)G.cell[2];
+    const float r2xy = G.r2xy;
+
+    int jx0 = (int)floorf((bxlo - Rx) / cxf), jx1 = (int)floorf((bxhi + Rx) / cxf);
+    int jy0 = (int)floorf((bylo - Ry) / cyf), jy1 = (int)floorf((byhi + Ry) / cyf);
+    if (pbc) {
+        jx0 = max(jx0, -nx); jx1 = min(jx1, 2 * nx - 1);
+        jy0 = max(jy0, -ny); jy1 = min(jy1, 2 * ny - 1);
+    } else {
+        jx0 = max(jx0, 0); jx1 = min(jx1, nx - 1);
+        jy0 = max(jy0, 0); jy1 = min(jy1, ny - 1);
+    }
+    jx0 = uni(jx0); jx1 = uni(jx1); jy0 = uni(jy0); jy1 = uni(jy1);
+
+    // lane-side weights
+    double lw[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const int nw = L.nw;
+    if (F64) {
+        for (int c = 0; c < nw && c < 5; c++) lw[c] = L.lane.w[p * nw + c];
+    }
+    const double* __restrict__ SX = L.strm.x;
+    const double* __restrict__ SY = L.strm.y;
+    const double* __restrict__ SZ = L.strm.z;
+    const double* __restrict__ SW = L.strm.w;
+    const uint32_t* __restrict__ CS = L.strm.cell_start;
+    const int wfunc = L.wfunc;
+    const double pimax2 = (MODE == kModeProj) ? gedges_p[0] : 0.0;
+    const uint32_t shr = L.br.shift, bshr = L.br.basesh, nbr1 = L.br.nb - 1;
+    const uint32_t shp = L.bp.shift, bshp = L.bp.basesh, nbp1 = L.bp.nb - 1;
+    const uint32_t trash_r = (uint32_t)Kr * step_r;  // projected: pairs beyond pi_max
+
+    uint64_t evals = 0;
+    // Shared (per-wave) uint32 histograms can receive up to 64 increments per
+    // stream point; flush them into the group's uint64 row before they could wrap.
+    const int shared_bins = (MODE == kModeXYZ) ? Kr * Kp : Kr;
+    const int row_len = (MODE == kModeXYZ) ? Kp + 1 : 1;
+    uint64_t* const shared_out = L.pcount + g * (int64_t)shared_bins;
+    uint32_t* const ch32 = reinterpret_cast<uint32_t*>(wsm + A.cnt_off);
+    uint64_t bound = 0;
+    bool flushed = false;
+
+    // one pair: (qx, qy, qz) is the stream point, j its sorted index
+    auto pair = [&](double qx, double qy, double qz, int j, double shx, double shy, double shz, int sxs, int sys,
+                    int szs, double ax, double ay, double az) __attribute__((always_inline)) {
+        double dx, dy, dz;
+        if (!SWAP) {
+            dx = ax - qx;
+            dy = ay - qy;
+            dz = az - qz;
+        } else {
+            if (sxs) qx = qx - shx;
+            if (sys) qy = qy - shy;
+            if (szs) qz = qz - shz;
+            dx = qx - px;
+            dy = qy - py;
+            dz = qz - pz;
+        }
+        if (MODE == kModeCount3D) {
+            const double dsq = dx * dx + dy * dy + dz * dz;
+            const uint32_t a = bin_addr<MULTI>(dsq, s_tr, shr, bshr, nbr1, step_r, gedges_r);
+            __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(hl + a), 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else if (MODE == kModeProj) {
+            const double dxy = dx * dx + dy * dy;
+            const double dz2 = dz * dz;
+            uint32_t a = bin_addr<MULTI>(dxy, s_tr, shr, bshr, nbr1, step_r, gedges_r);
+            a = (dz2 <= pimax2) ? a : trash_r;
+            __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(hl + a), 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else if (MODE == kModeXYZ) {
+            const double dxy = dx * dx + dy * dy;
+            const double dz2 = dz * dz;
+            const uint32_t a = bin_addr<MULTI>(dxy, s_tr, shr, bshr, nbr1, step_r, gedges_r);
+            const uint32_t b = bin_addr<MULTI>(dz2, s_tp, shp, bshp, nbp1, 4u, gedges_p);
+            __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(hl + a + b), 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            const double dsq = dx * dx + dy * dy + dz * dz;
+            const uint32_t a = bin_addr<MULTI>(dsq, s_tr, shr, bshr, nbr1, step_r, gedges_r);
+            double v;
+            if (MODE == kModeJac) {
+                v = SW[j];
+            } else {
+                double sw[5];
+                for (int c = 0; c < nw && c < 5; c++) sw[c] = SW[(int64_t)j * nw + c];
+                v = wfunc_eval(wfunc, lw, sw);
+            }
+            __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a), v, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (ALSO) {
+                __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(wsm + A.cnt_off + ((a + lane_off) >> 1)), 1u,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+    };
+
+    for (int jx = jx0; jx <= jx1; jx++) {
+        const float gx0 = fmaxf(0.f, fmaxf((float)jx * cxf - bxhi, bxlo - (float)(jx + 1) * cxf) - eps);
+        const int tx = jx < 0 ? -1 : (jx >= nx ? 1 : 0);
+        const int wx = jx - tx * nx;
+        for (int jy = jy0; jy <= jy1; jy++) {
+            const float gy0 = fmaxf(0.f, fmaxf((float)jy * cyf - byhi, bylo - (float)(jy + 1) * cyf) - eps);
+            const float d2 = gx0 * gx0 + gy0 * gy0;
+            if (d2 > r2xy) continue;
+            const float dzr = (G.shape == 0) ? sqrtf(fmaxf(r2xy - d2, 0.f)) + eps : Rz;
+            int jz0 = (int)floorf((bzlo - dzr) / czf), jz1 = (int)floorf((bzhi + dzr) / czf);
+            if (pbc) { jz0 = max(jz0, -nz); jz1 = min(jz1, 2 * nz - 1); }
+            else { jz0 = max(jz0, 0); jz1 = min(jz1, nz - 1); }
+            jz0 = uni(jz0); jz1 = uni(jz1);
+            if (jz0 > jz1) continue;
+            const int ty = jy < 0 ? -1 : (jy >= ny ? 1 : 0);
+            const int wy = jy - ty * ny;
+            const int64_t colbase = cell_off + ((int64_t)wx * ny + wy) * nz;
+            // up to three periodic pieces of the z-run
+            for (int piece = -1; piece <= 1; piece++) {
+                int u0, u1;
+                if (piece == -1) { u0 = jz0; u1 = min(jz1, -1); }
+                else if (piece == 0) { u0 = max(jz0, 0); u1 = min(jz1, nz - 1); }
+                else { u0 = max(jz0, nz); u1 = jz1; }
+                if (u0 > u1) continue;
+                const int tz = piece;
+                const int w0 = u0 - tz * nz, w1 = u1 - tz * nz;
+                const int jb = uni((int)CS[colbase + w0]);
+                const int je = uni((int)CS[colbase + w1 + 1]);
+                if (jb >= je) continue;
+                evals += (uint64_t)cnt * (uint64_t)(je - jb);
+                if (shared) {
+                    bound += (uint64_t)cnt * (uint64_t)(je - jb);
+                    if (bound >= 0xF0000000ull) {
+                        for (int s = lane; s < shared_bins; s += 64) {
+                            const int src = (MODE == kModeXYZ) ? (s / Kp) * row_len + s % Kp : s;
+                            shared_out[s] = (flushed ? shared_out[s] : 0ull) + ch32[src];
+                        }
+                        for (int s = lane; s < (Kr + 1) * row_len; s += 64) ch32[s] = 0u;
+                        flushed = true;
+                        bound = (uint64_t)cnt * (uint64_t)(je - jb);
+                    }
+                }
+                // x2shift of the reference: the shift that brings the sample-2
+                // point next to the sample-1 point.  Normal orientation visits
+                // sample-2 images (x2shift = t*L); swapped orientation visits
+                // sample-1 images (x2shift = -t*L).
+                const int sxs = SWAP ? -tx : tx, sys = SWAP ? -ty : ty, szs = SWAP ? -tz : tz;
+                const double shx = sxs < 0 ? -Lx : (sxs > 0 ? Lx : 0.0);
+                const double shy = sys < 0 ? -Ly : (sys > 0 ? Ly : 0.0);
+                const double shz = szs < 0 ? -Lz : (szs > 0 ? Lz : 0.0);
+                double ax = px, ay = py, az = pz;  // normal: x1tmp of the lane point
+                if (!SWAP) {
+                    if (sxs) ax = px - shx;
+                    if (sys) ay = py - shy;
+                    if (szs) az = pz - shz;
+                }
+                int j = jb;
+                for (; j + 4 <= je; j += 4) {
+                    double bx[4], by[4], bz[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        bx[u] = SX[j + u];
+                        by[u] = SY[j + u];
+                        bz[u] = SZ[j + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+                        pair(bx[u], by[u], bz[u], j + u, shx, shy, shz, sxs, sys, szs, ax, ay, az);
+                }
+                for (; j < je; j++) pair(SX[j], SY[j], SZ[j], j, shx, shy, shz, sxs, sys, szs, ax, ay, az);
+            }
+        }
+    }
+
+    // ---- group epilogue (the trash row Kr is dropped)
+    if (shared) {
+        for (int s = lane; s < shared_bins; s += 64) {
+            const int src = (MODE == kModeXYZ) ? (s / Kp) * row_len + s % Kp : s;
+            shared_out[s] = (flushed ? shared_out[s] : 0ull) + ch32[src];
+        }
+    } else if (MODE == kModeCount3D || MODE == kModeProj || ALSO) {
+        uint64_t* out = L.pcount + g * Kr;
+        for (int s = 0; s < Kr; s++) {
+            const uint64_t t = wave_sum_u64((uint64_t)ch32[s * 64 + lane]);
+            if (lane == 0) out[s] = t;
+        }
+    }
+    const double* fh = reinterpret_cast<const double*>(wsm);
+    if (MODE == kModeWSum) {
+        if (L.psum) {
+            for (int s = 0; s < Kr; s++) {
+                const double t = wave_sum_f64(fh[s * 64 + lane]);
+                if (lane == 0) L.psum[g * Kr + s] = t;
+            }
+        }
+    }
+    if (MODE == kModeJac) {
+        if (L.rows && act) {
+            const int64_t orig = (int64_t)L.lane.perm[p];
+            double* row = L.rows + orig * Kr;
+            for (int s = 0; s < Kr; s++) row[s] = fh[s * 64 + lane];
+        }
+        if (L.psum && !SWAP) {
+            for (int s = 0; s < Kr; s++) {
+                const double t = wave_sum_f64(act ? lw[0] * fh[s * 64 + lane] : 0.0);
+                if (lane == 0) L.psum[g * Kr + s] = t;
+            }
+        }
+    }
+    if (L.pevals && lane == 0) L.pevals[g] = evals;
+}
+
+template <int MODE, int SWAP, int ALSO, int MULTI, int WAVES>
+static int launch_t(const PairArgs& A, hipStream_t s) {
+    const int64_t nblocks = (A.L.ngroups + WAVES - 1) / WAVES;
+    if (nblocks <= 0) return 0;
+    hipLaunchKernelGGL((k_pairs<MODE, SWAP, ALSO, MULTI, WAVES>), dim3((unsigned)nblocks), dim3(WAVES * 64),
+                       (size_t)A.smem, s, A);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int MODE, int SWAP, int ALSO, int MULTI>
+static int launch_w(const PairArgs& A, int waves, hipStream_t s) {
+    if (waves == 8) return launch_t<MODE, SWAP, ALSO, MULTI, 8>(A, s);
+    if (waves == 4) return launch_t<MODE, SWAP, ALSO, MULTI, 4>(A, s);
+    if (waves == 2) return launch_t<MODE, SWAP, ALSO, MULTI, 2>(A, s);
+    return launch_t<MODE, SWAP, ALSO, MULTI, 1>(A, s);
+}
+
+template <int MODE, int SWAP, int ALSO>
+static int launch_m(const PairArgs& A, int waves, int multi, hipStream_t s) {
+    if (multi) return launch_w<MODE, SWAP, ALSO, 1>(A, waves, s);
+    return launch_w<MODE, SWAP, ALSO, 0>(A, waves, s);
+}
+
+static int align16(int v) { return (v + 15) & ~15; }
+
+int launch_pairs(const PairLaunch& L, hipStream_t s) {
+    PairArgs A;
+    A.L = L;
+    const int Kr = L.br.nbins, Kp = L.bp.nbins;
+    A.shared_count = 0;
+    A.cnt_off = 0;
+    int wave_bytes = 0;
+    switch (L.mode) {
+        case kModeCount3D:
+        case kModeProj:
+            if (Kr > 64) { A.shared_count = 1; wave_bytes = (Kr + 1) * 4; }
+            else wave_bytes = (Kr + 1) * 64 * 4;
+            break;
+        case kModeXYZ: wave_bytes = (Kr + 1) * (Kp + 1) * 4; break;
+        case kModeWSum:
+        case kModeJac:
+            A.cnt_off = (Kr + 1) * 64 * 8;
+            wave_bytes = A.cnt_off + (L.also_count ? (Kr + 1) * 64 * 4 : 0);
+            break;
+        default: return -1;
+    }
+    if (L.br.nb > kTableMax || L.bp.nb > kTableMax) return -1;
+    A.wave_bytes = align16(wave_bytes);
+    const int static_lds = (int)sizeof(LdsEntry) * kTableMax * (L.mode == kModeXYZ ? 2 : 1);
+    int waves = 8;
+    while (waves > 1 && static_lds + waves * A.wave_bytes > 72 * 1024) waves >>= 1;
+    A.smem = waves * A.wave_bytes;
+    if (static_lds + A.smem > 160 * 1024) return -2;
+    const int multi = (L.br.multi || L.bp.multi) ? 1 : 0;
+    const int also = L.also_count ? 1 : 0;
+    switch (L.mode) {
+        case kModeCount3D: return launch_m<kModeCount3D, 0, 0>(A, waves, multi, s);
+        case kModeProj: return launch_m<kModeProj, 0, 0>(A, waves, multi, s);
+        case kModeXYZ: return launch_m<kModeXYZ, 0, 0>(A, waves, multi, s);
+        case kModeWSum:
+            return also ? launch_m<kModeWSum, 0, 1>(A, waves, multi, s) : launch_m<kModeWSum, 0, 0>(A, waves, multi, s);
+        case kModeJac:
+            if (L.swap)
+                return also ? launch_m<kModeJac, 1, 1>(A, waves, multi, s) : launch_m<kModeJac, 1, 0>(A, waves, multi, s);
+            return also ? launch_m<kModeJac, 0, 1>(A, waves, multi, s) : launch_m<kModeJac, 0, 0>(A, waves, multi, s);
+    }
+    return -1;
+}
+
+}  // namespace hp

@@ -1,0 +1,178 @@
+// reduce.hip -- fixed-order reductions of per-group partial histograms into
+// the reference's cumulative outputs, and the backward contraction.
+//
+// The reference sums per-process count vectors with np.sum(..., axis=0)
+// (PC/npairs_3d.py:142-146) and returns cumulative counts N(<= r_k) because its
+// engines walk the bins from the top down (PC/cpairs/npairs_3d_engine.pyx:178-182).
+// Here each group contributed the histogram of the *first* bin a pair falls in;
+// an inclusive prefix sum over bins (2-D for xy_z) turns it into the same
+// cumulative counts.  All sums run in a fixed order, so results are
+// run-to-run bitwise reproducible, including the FP64 marked sums.
+#include "hp_internal.h"
+
+namespace hp {
+
+constexpr int kChunk = 512;  // groups per stage-1 block
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_reduce_s1(const T* __restrict__ partial, int nb, const MockGeom* __restrict__ geom,
+                                                   int side, int nchunk_max, T* __restrict__ s1) {
+    __shared__ T sh[256];
+    const int m = blockIdx.y, c = blockIdx.x;
+    const int64_t go = geom[m].group_off[side], ng = geom[m].ngroups[side];
+    const int64_t r0 = (int64_t)c * kChunk;
+    const int64_t r1 = r0 + kChunk < ng ? r0 + kChunk : ng;
+    T* out = s1 + ((int64_t)m * nchunk_max + c) * nb;
+    const int nbt = nb < 256 ? nb : 256;
+    const int rstride = 256 / nbt;
+    for (int b0 = 0; b0 < nb; b0 += nbt) {
+        const int t = threadIdx.x;
+        const int b = b0 + t % nbt;
+        const int rr = t / nbt;
+        T acc = 0;
+        if (rr < rstride && b < nb) {
+            for (int64_t r = r0 + rr; r < r1; r += rstride) acc += partial[(go + r) * nb + b];
+        }
+        sh[t] = acc;
+        __syncthreads();
+        if (t < nbt && b0 + t < nb) {
+            T tot = 0;
+            for (int q = 0; q < rstride; q++) tot += sh[q * nbt + t];
+            out[b0 + t] = tot;
+        }
+        __syncthreads();
+    }
+}
+
+// Stage 2: per mock, sum the chunk rows in order, then the inclusive prefix.
+// dims: nr (and np for the 2-D xy_z histogram; np = 0 means 1-D).
+template <typename T, typename TO>
+__global__ void __launch_bounds__(256) k_reduce_s2(const T* __restrict__ s1, int nr, int np, const MockGeom* __restrict__ geom,
+                                                   int side, int nchunk_max, TO* __restrict__ out) {
+    const int m = blockIdx.x;
+    const int nb = np > 0 ? nr * np : nr;
+    const int64_t ng = geom[m].ngroups[side];
+    const int nchunks = (int)((ng + kChunk - 1) / kChunk);
+    TO* o = out + (int64_t)m * nb;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        T tot = 0;
+        for (int c = 0; c < nchunks; c++) tot += s1[((int64_t)m * nchunk_max + c) * nb + b];
+        o[b] = (TO)tot;
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (np <= 0) {
+        if (threadIdx.x == 0) {
+            TO run = 0;
+            for (int k = 0; k < nr; k++) { run += o[k]; o[k] = run; }
+        }
+    } else {
+        for (int k = threadIdx.x; k < nr; k += blockDim.x) {
+            TO run = 0;
+            for (int q = 0; q < np; q++) { run += o[k * np + q]; o[k * np + q] = run; }
+        }
+        __threadfence_block();
+        __syncthreads();
+        for (int q = threadIdx.x; q < np; q += blockDim.x) {
+            TO run = 0;
+            for (int k = 0; k < nr; k++) { run += o[k * np + q]; o[k * np + q] = run; }
+        }
+    }
+}
+
+// Plain per-mock totals (no prefix): used for the candidate-evaluation counter.
+__global__ void __launch_bounds__(256) k_reduce_plain(const uint64_t* __restrict__ s1, const MockGeom* __restrict__ geom,
+                                                      int side, int nchunk_max, int64_t* __restrict__ out) {
+    const int m = blockIdx.x;
+    const int64_t ng = geom[m].ngroups[side];
+    const int nchunks = (int)((ng + kChunk - 1) / kChunk);
+    if (threadIdx.x == 0) {
+        uint64_t tot = 0;
+        for (int c = 0; c < nchunks; c++) tot += s1[(int64_t)m * nchunk_max + c];
+        out[m] = (int64_t)tot;
+    }
+}
+
+static int64_t nchunk_of(int64_t max_groups) { return (max_groups + kChunk - 1) / kChunk; }
+
+int64_t reduce_scratch_elems(int64_t max_groups_per_mock, int nmocks, int nb) {
+    const int64_t nc = nchunk_of(max_groups_per_mock) > 0 ? nchunk_of(max_groups_per_mock) : 1;
+    return nc * nmocks * (int64_t)nb;
+}
+
+int launch_reduce_counts_s(const uint64_t* partial, int nr, int np, const MockGeom* geom, int nmocks, int side,
+                           int64_t max_groups, uint64_t* scratch, int64_t* counts, hipStream_t s) {
+    const int nb = np > 0 ? nr * np : nr;
+    const int nc = (int)(nchunk_of(max_groups) > 0 ? nchunk_of(max_groups) : 1);
+    hipLaunchKernelGGL(k_reduce_s1<uint64_t>, dim3(nc, nmocks), dim3(256), 0, s, partial, nb, geom, side, nc, scratch);
+    hipLaunchKernelGGL((k_reduce_s2<uint64_t, int64_t>), dim3(nmocks), dim3(256), 0, s, scratch, nr, np, geom, side, nc,
+                       counts);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_reduce_sums_s(const double* partial, int nr, const MockGeom* geom, int nmocks, int side, int64_t max_groups,
+                         double* scratch, double* sums, hipStream_t s) {
+    const int nc = (int)(nchunk_of(max_groups) > 0 ? nchunk_of(max_groups) : 1);
+    hipLaunchKernelGGL(k_reduce_s1<double>, dim3(nc, nmocks), dim3(256), 0, s, partial, nr, geom, side, nc, scratch);
+    hipLaunchKernelGGL((k_reduce_s2<double, double>), dim3(nmocks), dim3(256), 0, s, scratch, nr, 0, geom, side, nc,
+                       sums);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_reduce_evals_s(const uint64_t* pevals, const MockGeom* geom, int nmocks, int side, int64_t max_groups,
+                          uint64_t* scratch, int64_t* evals, hipStream_t s) {
+    const int nc = (int)(nchunk_of(max_groups) > 0 ? nchunk_of(max_groups) : 1);
+    hipLaunchKernelGGL(k_reduce_s1<uint64_t>, dim3(nc, nmocks), dim3(256), 0, s, pevals, 1, geom, side, nc, scratch);
+    hipLaunchKernelGGL(k_reduce_plain, dim3(nmocks), dim3(64), 0, s, scratch, geom, side, nc, evals);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ------------------------------------------------------------- backward
+constexpr int kDotMocks = 240;
+struct DotArgs {
+    const double* rows;
+    const double* g;
+    double* grad;
+    int nr;
+    int nm;           // mocks in this launch
+    int m0;           // first mock of this launch (row of g)
+    int64_t off[kDotMocks + 1];
+};
+
+// grad[p] = sum_s rows[p][s] * g[mock(p)][s]: one thread per point, the
+// contraction of the per-point Jacobian rows with the suffix-summed upstream
+// gradient.  HBM-bound (reads nr doubles per point).
+__global__ void __launch_bounds__(256) k_rows_dot(const DotArgs a) {
+    const int64_t p = a.off[0] + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= a.off[a.nm]) return;
+    int lo = 0, hi = a.nm - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.off[mid] <= p) lo = mid; else hi = mid - 1;
+    }
+    const double* r = a.rows + p * a.nr;
+    const double* g = a.g + (int64_t)(a.m0 + lo) * a.nr;
+    double acc = 0.0;
+    for (int s = 0; s < a.nr; s++) acc += r[s] * g[s];
+    a.grad[p] = acc;
+}
+
+int launch_rows_dot(const double* rows, int nr, const double* gsuffix, const int64_t* offsets, int nmocks,
+                    double* grad, hipStream_t s) {
+    for (int m0 = 0; m0 < nmocks; m0 += kDotMocks) {
+        DotArgs a;
+        a.rows = rows;
+        a.g = gsuffix;
+        a.grad = grad;
+        a.nr = nr;
+        a.nm = nmocks - m0 < kDotMocks ? nmocks - m0 : kDotMocks;
+        a.m0 = m0;
+        for (int i = 0; i <= a.nm; i++) a.off[i] = offsets[m0 + i];
+        const int64_t n = a.off[a.nm] - a.off[0];
+        if (n <= 0) continue;
+        hipLaunchKernelGGL(k_rows_dot, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace hp

@@ -1,0 +1,147 @@
+/*
+ * halopair.h -- C ABI of the MI355X pair-counting engine (libhalopair.so).
+ *
+ * Drop-in replacement for the native engines behind
+ * halotools.mock_observables.{npairs_3d, npairs_projected, npairs_xy_z,
+ * marked_npairs_3d}.  The reference binds its engines as Cython `def`
+ * functions that take a RectangularDoubleMesh plus the six coordinate arrays
+ * and return a fresh NumPy array; here the mesh is built on the GPU and every
+ * buffer is caller-owned device memory passed as a plain pointer.
+ *
+ * Conventions
+ *   - All data pointers are device pointers (hipMalloc / torch tensors) unless
+ *     marked "host".  The library never frees or retains a caller pointer after
+ *     the call returns.
+ *   - Every entry point only enqueues work on `stream`: no host<->device copy,
+ *     no allocation, no synchronisation.  The calls can therefore be captured
+ *     into a HIP graph.  Small host-side descriptions (bin edges, mock table)
+ *     are carried into the workspace by kernel arguments.
+ *   - Scratch memory is the caller's `workspace` of at least
+ *     hp_workspace_size() bytes.  Two calls may share a workspace only if they
+ *     are ordered on one stream.
+ *   - Return value: 0 on success, negative on error; hp_last_error() then
+ *     returns a thread-local, human-readable message.
+ *   - Positions are FP64.  Counts are computed with the reference's exact FP64
+ *     expression order (dx = (x1 - shift) - x2; dsq = dx*dx + dy*dy + dz*dz,
+ *     no fused multiply-add), so integer counts are bit-identical to the
+ *     reference engine on the same inputs.
+ *   - Periodic boxes require every coordinate in [0, period]; a coordinate
+ *     outside it (or non-finite) is reported through the `status` output.
+ *
+ * PC/ abbreviates halotools/mock_observables/pair_counters/ in the reference.
+ */
+#ifndef HALOPAIR_H
+#define HALOPAIR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HP_ABI_VERSION 1
+#define HP_MAX_BINS 512 /* per axis */
+
+/* Which reference engine a request replaces. */
+enum hp_kind {
+    HP_NPAIRS_3D = 0,        /* PC/cpairs/npairs_3d_engine.pyx:17            */
+    HP_NPAIRS_PROJECTED = 1, /* PC/cpairs/npairs_projected_engine.pyx:17     */
+    HP_NPAIRS_XY_Z = 2,      /* PC/cpairs/npairs_xy_z_engine.pyx:17          */
+    HP_MARKED_NPAIRS_3D = 3  /* PC/marked_cpairs/marked_npairs_3d_engine.pyx:22 */
+};
+
+/* Bit flags of hp_request.flags */
+enum hp_flags {
+    HP_SAME_SAMPLES = 1, /* sample2 arrays are sample1's (auto-count)                 */
+    HP_ROWS1 = 2,        /* marked id 1: write rows1 = d sums / d weights1 per point   */
+    HP_ROWS2 = 4,        /* marked id 1: write rows2 = d sums / d weights2 per point   */
+    HP_NO_SUMS = 8,      /* marked: skip the weighted sums (rows only)                */
+    HP_ALSO_COUNTS = 16  /* marked: also produce the unweighted counts of npairs_3d   */
+};
+
+/* One realisation ("mock") of a batched request.  A single call is nmocks=1. */
+typedef struct hp_mock {
+    int64_t n1;        /* points of sample 1 in this mock                          */
+    int64_t n2;        /* points of sample 2 (ignored with HP_SAME_SAMPLES)        */
+    double period[3];  /* periodic box lengths; for pbc=0 the enclosing box Lbox   */
+    int32_t pbc;       /* 1: periodic boundary conditions                          */
+    int32_t reserved;
+    double cell[3];    /* fine-cell edge of the GPU mesh; <= 0 selects the default */
+} hp_mock;
+
+typedef struct hp_request {
+    int32_t kind;              /* enum hp_kind                                         */
+    int32_t flags;             /* enum hp_flags                                        */
+    int32_t nmocks;            /* >= 1                                                 */
+    int32_t wfunc;             /* marked: weight_func_id 1..17                         */
+    const hp_mock* mocks;      /* host [nmocks]; sample points of mock m follow those  */
+                               /* of mock m-1 in the coordinate arrays                 */
+    const double *x1, *y1, *z1; /* sample 1, SoA, n1 total                              */
+    const double *x2, *y2, *z2; /* sample 2, SoA, n2 total (= sample 1 if SAME_SAMPLES) */
+    const double* w1;          /* marked: weights1, row-major (n1_total, nw)           */
+    const double* w2;          /* marked: weights2, row-major (n2_total, nw)           */
+    int32_t nw;                /* weights per point (id 1,2: 1; 3-11: 2; 12,13: 4; ...) */
+    int32_t nr;                /* number of radial / rp edges                          */
+    int32_t np;                /* xy_z: number of pi edges; projected: 1               */
+    int32_t reserved;
+    const double* r2;          /* host [nr]: rbins*rbins (or rp_bins*rp_bins)          */
+    const double* p2;          /* host [np]: pi_bins*pi_bins, or {pi_max*pi_max}        */
+    double search[3];          /* mesh search lengths (rmax,rmax,rmax) / (rp,rp,pi)     */
+    /* outputs, device */
+    int64_t* counts;           /* [nmocks][nr] cumulative (xy_z: [nmocks][nr][np])      */
+    double* sums;              /* marked: [nmocks][nr] cumulative weighted sums         */
+    double* rows1;             /* HP_ROWS1: [n1_total][nr] per-bin (not cumulative)     */
+    double* rows2;             /* HP_ROWS2: [n2_total][nr]                              */
+    int64_t* evals;            /* optional [nmocks]: candidate pairs evaluated          */
+    int32_t* status;           /* optional [1]: 0, or 1 when a point is outside its box */
+    void* workspace;           /* device scratch                                       */
+    size_t workspace_bytes;
+    void* stream;              /* hipStream_t; NULL = default stream                   */
+} hp_request;
+
+/* Thread-local message describing the last error (empty string if none). */
+const char* hp_last_error(void);
+
+/* ABI version of the loaded library (HP_ABI_VERSION). */
+int hp_abi_version(void);
+
+/* Bytes of workspace the request needs (pointers in `req` are not read). */
+size_t hp_workspace_size(const hp_request* req);
+
+/* npairs_3d: cumulative int64 counts[m][k] of ordered pairs with
+ * (x1-shift-x2)^2+... <= r2[k].  Replaces npairs_3d_engine
+ * (PC/cpairs/npairs_3d_engine.pyx:17-184) and the RectangularDoubleMesh build
+ * (PC/rectangular_mesh.py:228-374) that feeds it. */
+int hp_npairs_3d(const hp_request* req);
+
+/* npairs_projected: counts[m][k] of pairs with dxy^2 <= r2[k] and
+ * dz^2 <= p2[0].  Replaces PC/cpairs/npairs_projected_engine.pyx:17-192. */
+int hp_npairs_projected(const hp_request* req);
+
+/* npairs_xy_z: counts[m][k][g] of pairs with dxy^2 <= r2[k], dz^2 <= p2[g].
+ * Replaces PC/cpairs/npairs_xy_z_engine.pyx:17-196. */
+int hp_npairs_xy_z(const hp_request* req);
+
+/* marked_npairs_3d: sums[m][k] of wfunc(w1_i, w2_j) over pairs within r2[k].
+ * Replaces PC/marked_cpairs/marked_npairs_3d_engine.pyx:22-218 together with
+ * the marking functions of PC/marked_cpairs/marking_functions.pyx:14-217.
+ * With wfunc == 1 and HP_ROWS1/HP_ROWS2 it also writes, for every point, the
+ * per-bin derivative of the (non-cumulative) bin sums with respect to that
+ * point's weight: rows1[i][s] = sum of w2_j over partners j in bin s.  This is
+ * the Jacobian the backward pass contracts (no counterpart in the reference). */
+int hp_marked_npairs_3d(const hp_request* req);
+
+/* Backward of the product-weighted marked sums (no counterpart in the
+ * reference): grad[p] = sum_s rows[p][s] * gsuffix[mock(p)][s], where
+ * gsuffix[m][s] = sum_{k>=s} dL/dsums[m][k].  `offsets` (host, nmocks+1)
+ * delimits each mock's points (rows has offsets[nmocks] rows); rows, gsuffix
+ * and grad are device arrays. */
+int hp_marked_backward(const double* rows, int32_t nr, const double* gsuffix,
+                       const int64_t* offsets, int32_t nmocks, double* grad, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HALOPAIR_H */

@@ -1,0 +1,310 @@
+"""GPU parity: the HIP pair counters against the oracle and the reference's own
+known answers.
+
+Counts must be bit-exact (integer equality) with the oracle, which restates the
+reference engines loop for loop; marked sums must agree to 1e-12 relative.
+The known-answer cases restate the reference tests they cite
+(PC/test_pair_counters/..., PC = halotools/mock_observables/pair_counters).
+"""
+import numpy as np
+import pytest
+
+from oracle import mesh as ORC
+from tests.helpers import (fixed_seed, generate_3d_regular_mesh, generate_locus_of_3d_points, grid_rbins,
+                           random_sample)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mo(gpu):
+    import halotools_amd.mock_observables as mo
+    return mo
+
+
+# --------------------------------------------------------------- known answers
+def test_tight_locus_pbc_irrelevant(mo):
+    # PC/test_pair_counters/test_npairs_3d.py:33-44
+    d1 = generate_locus_of_3d_points(100, xc=0.1, yc=0.1, zc=0.1, seed=fixed_seed)
+    d2 = generate_locus_of_3d_points(100, xc=0.1, yc=0.1, zc=0.2, seed=fixed_seed)
+    res = mo.npairs_3d(d1, d2, np.array((0.05, 0.15, 0.3)), period=1)
+    assert np.all(res == [0, 100 * 100, 100 * 100])
+
+
+def test_tight_locus_pbc_relevant(mo):
+    # PC/test_pair_counters/test_npairs_3d.py:47-60
+    d1 = generate_locus_of_3d_points(100, xc=0.1, yc=0.1, zc=0.05, seed=fixed_seed)
+    d2 = generate_locus_of_3d_points(100, xc=0.1, yc=0.1, zc=0.95, seed=fixed_seed)
+    res = mo.npairs_3d(d1, d2, np.array((0.05, 0.15, 0.3)), period=1)
+    assert np.all(res == [0, 100 * 100, 100 * 100])
+
+
+@pytest.mark.parametrize("kw", [
+    dict(num_threads="max"), dict(num_threads=1), dict(period=1.), dict(approx_cell1_size=[0.1, 0.1, 0.1]),
+    dict(approx_cell1_size=[0.1, 0.1, 0.1], approx_cell2_size=[0.1, 0.1, 0.1]),
+    dict(approx_cell1_size=0.1, approx_cell2_size=0.1, period=1),
+    dict(approx_cell1_size=[0.2, 0.2, 0.2], approx_cell2_size=[0.15, 0.15, 0.15], period=1),
+])
+def test_tight_locus_cell_size_variants(mo, kw):
+    # PC/test_pair_counters/test_npairs_3d.py:63-177
+    p1 = generate_locus_of_3d_points(100, xc=0.1, yc=0.1, zc=0.1, seed=fixed_seed)
+    p2 = generate_locus_of_3d_points(200, xc=0.1, yc=0.1, zc=0.25, seed=fixed_seed)
+    res = mo.npairs_3d(p1, p2, np.array([0.1, 0.2, 0.3]), **kw)
+    assert np.all(res == [0, 100 * 200, 100 * 200])
+
+
+def test_regular_grid_3d(mo):
+    # PC/test_pair_counters/test_npairs_3d.py:180-197
+    d = generate_3d_regular_mesh(10)
+    res = mo.npairs_3d(d, d, grid_rbins(), period=1., approx_cell1_size=0.1)
+    assert np.all(res == [1000, 7000, 19000, 27000])
+
+
+def test_regular_grid_projected(mo):
+    # PC/test_pair_counters/test_npairs_projected.py:222-265
+    d = generate_3d_regular_mesh(10)
+    res = mo.npairs_projected(d, d, grid_rbins(), 0.1 / 100., period=1., approx_cell1_size=0.1)
+    assert np.all(res == [1000, 5000, 9000, 9000])
+    res = mo.npairs_projected(d, d, grid_rbins(), 1.1 * 0.1, period=1., approx_cell1_size=0.1)
+    assert np.all(res == [3000, 15000, 27000, 27000])
+
+
+def test_regular_grid_xy_z(mo):
+    # PC/test_pair_counters/test_npairs_xy_z.py:175-198
+    d = generate_3d_regular_mesh(10)
+    res = mo.npairs_xy_z(d, d, grid_rbins(), np.array([0, 1.1 * 0.1]), period=1., approx_cell1_size=0.1)
+    assert np.all(res[:, 0] == [1000, 5000, 9000, 9000])
+    assert np.all(res[:, 1] == [3000, 15000, 27000, 27000])
+
+
+def test_projected_loci(mo):
+    # PC/test_pair_counters/test_npairs_projected.py:20-107
+    d1 = generate_locus_of_3d_points(100, xc=0.1, yc=0.1, zc=0.1, seed=fixed_seed)
+    d2 = generate_locus_of_3d_points(110, xc=0.1, yc=0.1, zc=0.2, seed=fixed_seed)
+    rp = np.array((0.05, 0.15, 0.3))
+    assert np.all(mo.npairs_projected(d1, d2, rp, 0.05, period=1) == [0, 0, 0])
+    assert np.all(mo.npairs_projected(d1, d2, rp, 0.15, period=1) == [11000] * 3)
+    d1 = generate_locus_of_3d_points(100, xc=0.1, yc=0.05, zc=0.1, seed=fixed_seed)
+    d2 = generate_locus_of_3d_points(300, xc=0.1, yc=0.95, zc=0.1, seed=fixed_seed)
+    assert np.all(mo.npairs_projected(d1, d2, rp, 0.1, period=1) == [0, 30000, 30000])
+    assert np.all(mo.npairs_projected(d1, d2, rp, 0.1) == [0, 0, 0])
+    d1 = generate_locus_of_3d_points(100, xc=0.1, yc=0.1, zc=0.05, seed=fixed_seed)
+    d2 = generate_locus_of_3d_points(101, xc=0.1, yc=0.2, zc=0.95, seed=fixed_seed)
+    assert np.all(mo.npairs_projected(d1, d2, rp, 0.2, period=1) == [0, 10100, 10100])
+    assert np.all(mo.npairs_projected(d1, d2, rp, 0.05, period=1) == [0, 0, 0])
+    assert np.all(mo.npairs_projected(d1, d2, rp, 0.2) == [0, 0, 0])
+
+
+# --------------------------------------------------------------- against the oracle
+@pytest.mark.parametrize("period", [np.array([1.0, 1.0, 1.0]), None])
+def test_brute_force_1000(mo, period):
+    # PC/test_pair_counters/test_npairs_3d.py:225-267
+    s = random_sample(1000)
+    rb = np.array([0.001, 0.1, 0.2, 0.3])
+    res = mo.npairs_3d(s, s, rb, period=period)
+    assert res.dtype == np.int64 and res.shape == (4,)
+    assert np.all(res == ORC.npairs_3d(s, s, rb, period=period))
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("period", [250.0, None])
+def test_cross_random(mo, seed, period):
+    rng = np.random.default_rng([43, seed])
+    s1 = rng.uniform(0, 250, (3000, 3))
+    s2 = rng.uniform(0, 250, (2500, 3))
+    rb = np.logspace(-1, np.log10(20), 15)
+    assert np.all(mo.npairs_3d(s1, s2, rb, period=period) == ORC.npairs_3d(s1, s2, rb, period=period))
+
+
+def test_config1_cross_20k(mo):
+    # 20,000 x 20,000 uniform in a periodic L=250 box, 15 log edges 0.1-20
+    rng = np.random.default_rng([43, 0])
+    s1 = rng.uniform(0, 250, (20000, 3))
+    s2 = rng.uniform(0, 250, (20000, 3))
+    rb = np.logspace(-1, np.log10(20), 15)
+    gpu = mo.npairs_3d(s1, s2, rb, period=250)
+    assert np.all(gpu == ORC.npairs_3d(s1, s2, rb, period=250, num_threads=8))
+
+
+def test_points_on_box_edge(mo):
+    # coordinates exactly 0 and exactly L (the reference clamps x == L into the last cell)
+    rng = np.random.default_rng(7)
+    s = rng.uniform(0, 10, (2000, 3))
+    s[:50, 0] = 10.0
+    s[50:100, 1] = 0.0
+    s[100:150, 2] = 10.0
+    s[150:160] = 10.0
+    rb = np.array([0.05, 0.5, 1.0, 2.0, 3.0])
+    assert np.all(mo.npairs_3d(s, s, rb, period=10) == ORC.npairs_3d(s, s, rb, period=10))
+
+
+def test_non_cubic_xy_z(mo):
+    # PC/test_pair_counters/test_non_cubic_volumes.py:17-68
+    rng = np.random.RandomState(fixed_seed)
+    period = [1, 2, 3]
+    x1, y1, z1 = rng.uniform(0, 1, 100), rng.uniform(0, 2, 100), rng.uniform(0, 3, 100)
+    x2, y2, z2 = rng.uniform(0, 1, 90), rng.uniform(0, 2, 90), rng.uniform(0, 3, 90)
+    d1, d2 = np.vstack((x1, y1, z1)).T, np.vstack((x2, y2, z2)).T
+    rp = np.arange(0, 0.31, 0.1)
+    pi = np.arange(0, 0.31, 0.1)
+    for per in (period, None):
+        assert np.all(mo.npairs_xy_z(d1, d2, rp, pi, period=per) == ORC.npairs_xy_z(d1, d2, rp, pi, period=per))
+
+
+@pytest.mark.parametrize("period", [1.0, None])
+def test_projected_and_xy_z_random(mo, period):
+    s1 = random_sample(1500, seed=5)
+    s2 = random_sample(1200, seed=6)
+    rp = np.array([0.001, 0.05, 0.1, 0.2, 0.3])
+    pi = np.arange(0, 0.31, 0.05)
+    assert np.all(mo.npairs_projected(s1, s2, rp, 0.1, period=period) ==
+                  ORC.npairs_projected(s1, s2, rp, 0.1, period=period))
+    assert np.all(mo.npairs_xy_z(s1, s2, rp, pi, period=period) ==
+                  ORC.npairs_xy_z(s1, s2, rp, pi, period=period))
+
+
+def test_two_element_non_monotone_bins(mo):
+    # two-element rbins are not checked for order by the reference (PC/npairs_3d.py:178-179);
+    # its top-down loop then counts a pair in both bins only if dsq <= rbins[1]**2
+    s = random_sample(800, seed=9)
+    for rb in (np.array([0.2, 0.1]), np.array([0.1, 0.1]), np.array([0.0, 0.3])):
+        assert np.all(mo.npairs_3d(s, s, rb, period=1.0) == ORC.npairs_3d(s, s, rb, period=1.0))
+
+
+def test_zero_edges_and_duplicates(mo):
+    # self-pairs sit exactly at dsq == 0 <= 0; duplicated points too
+    s = random_sample(500, seed=11)
+    s = np.concatenate([s, s[:100]])
+    rb = np.array([0.0, 0.01, 0.1, 0.3])
+    assert np.all(mo.npairs_3d(s, s, rb, period=1.0) == ORC.npairs_3d(s, s, rb, period=1.0))
+    pi = np.array([0.0, 0.1, 0.2])
+    assert np.all(mo.npairs_xy_z(s, s, rb, pi, period=1.0) == ORC.npairs_xy_z(s, s, rb, pi, period=1.0))
+
+
+def test_clustered_medium(mo):
+    # clustered auto-count: tight clumps make dense cells and many small separations
+    rng = np.random.default_rng(123)
+    centres = rng.uniform(0, 200, (4000, 3))
+    sats = np.repeat(centres, 5, axis=0) + rng.normal(0, 0.5, (20000, 3))
+    pts = np.mod(np.concatenate([centres, sats]), 200.0)
+    rb = np.logspace(-1, np.log10(10), 20)
+    assert np.all(mo.npairs_3d(pts, pts, rb, period=200) == ORC.npairs_3d(pts, pts, rb, period=200, num_threads=8))
+
+
+def test_float32_and_int_inputs(mo):
+    s = random_sample(700, seed=3).astype(np.float32)
+    rb = np.array([0.05, 0.1, 0.2])
+    for per in (1.0, None):
+        assert np.all(mo.npairs_3d(s, s, rb, period=per) == ORC.npairs_3d(s, s, rb, period=per))
+    si = (random_sample(700, seed=4) * 100).astype(np.int64)
+    rb = np.array([1.0, 5.0, 10.0])
+    assert np.all(mo.npairs_3d(si, si, rb) == ORC.npairs_3d(si, si, rb))
+
+
+def test_many_bins_shared_histogram(mo):
+    # more than 64 edges switches the kernel to one histogram per wavefront
+    s = random_sample(2000, seed=12)
+    rb = np.linspace(0.001, 0.3, 100)
+    assert np.all(mo.npairs_3d(s, s, rb, period=1.0) == ORC.npairs_3d(s, s, rb, period=1.0))
+
+
+def test_small_and_empty_samples(mo):
+    s = random_sample(10, seed=2)
+    rb = np.array([0.1, 0.2, 0.3])
+    assert np.all(mo.npairs_3d(s[:1], s, rb, period=1.0) == ORC.npairs_3d(s[:1], s, rb, period=1.0))
+    assert np.all(mo.npairs_3d(s[:0], s, rb, period=1.0) == 0)
+    assert np.all(mo.npairs_3d(s, s[:0], rb, period=1.0) == 0)
+
+
+def test_outside_box_raises(mo):
+    s = random_sample(100, seed=2)
+    s[3, 1] = 1.5
+    with pytest.raises(ValueError):
+        mo.npairs_3d(s, s, np.array([0.1, 0.2]), period=1.0)
+
+
+def test_device_tensor_in_out(mo, gpu):
+    import torch
+    s = random_sample(3000, seed=21)
+    rb = np.array([0.01, 0.05, 0.1, 0.2])
+    t = torch.from_numpy(s).to(gpu)
+    out = mo.npairs_3d(t, t, rb, period=1.0)
+    assert isinstance(out, torch.Tensor) and out.is_cuda and out.dtype == torch.int64
+    assert np.all(out.cpu().numpy() == ORC.npairs_3d(s, s, rb, period=1.0))
+
+
+def test_deterministic_repeats(mo):
+    rng = np.random.default_rng(99)
+    s = rng.uniform(0, 100, (20000, 3))
+    rb = np.logspace(-1, 1, 12)
+    a = mo.npairs_3d(s, s, rb, period=100)
+    for _ in range(3):
+        assert np.array_equal(a, mo.npairs_3d(s, s, rb, period=100))
+
+
+# --------------------------------------------------------------- marked
+def _weights(n, width, seed):
+    return np.random.RandomState(seed).random_sample(n * width).reshape(n, width) - 0.5
+
+
+@pytest.mark.parametrize("wid", list(range(1, 18)))
+def test_marked_all_wfuncs_vs_oracle(mo, wid):
+    from halotools_amd.mock_observables.pair_counters.marked_npairs_3d import _func_signature_int_from_wfunc
+    s = random_sample(1000)
+    width = _func_signature_int_from_wfunc(wid)
+    w = _weights(1000, width, fixed_seed)
+    # make the (in)equality based functions see some ties
+    w[:, 0] = np.round(w[:, 0] * 4) / 4 if wid in (3, 4, 16, 17) else w[:, 0]
+    if wid in (16, 17):
+        w[:, 4] = np.round(w[:, 4] * 2) / 2
+    rb = np.array([0.0, 0.1, 0.2, 0.3])
+    gpu = mo.marked_npairs_3d(s, s, rb, wid, period=1.0, weights1=w, weights2=w)
+    ref = ORC.marked_npairs_3d(s, s, rb, 1.0, w, w, wid)
+    np.testing.assert_allclose(gpu, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("period", [np.array([1.0, 1.0, 1.0]), None])
+def test_marked_product_random_weights(mo, period):
+    # PC/test_pair_counters/test_marked_npairs_3d.py:58-89 and :146-178 (rtol 1e-5 there)
+    s = random_sample(1000)
+    w = np.random.RandomState(fixed_seed).random_sample((1000, 1))
+    rb = np.array([0.0, 0.1, 0.2, 0.3])
+    gpu = mo.marked_npairs_3d(s, s, rb, 1, period=period, weights1=w, weights2=w)
+    ref = ORC.marked_npairs_3d(s, s, rb, period, w, w, 1)
+    np.testing.assert_allclose(gpu, ref, rtol=1e-12)
+
+
+def _grid_points():
+    # PC/test_pair_counters/test_marked_npairs_3d.py:40-55
+    epsilon = 0.001
+    g = np.linspace(0, 1 - epsilon, 10)
+    xx, yy, zz = np.array(np.meshgrid(g, g, g))
+    return np.vstack([xx.flatten(), yy.flatten(), zz.flatten()]).T, np.array([0.0, 0.1, 0.2, 0.3])
+
+
+@pytest.mark.parametrize("wid,w0,w1,factor", [
+    (1, 3, None, 9.0), (2, 3, None, 6.0), (3, 2, 3, 9.0), (3, 3, 2, 4.0), (4, 2, 3, 0.0), (5, 2, 3, 0.0),
+    (6, 2, 3, 0.0), (7, 1, -1, -1.0), (8, 1, 3, 3.0), (9, 1, 3, 3.0), (10, 1, 3, 0.0), (10, 1, -3, -3.0),
+])
+def test_marked_uniform_weight_identities(mo, wid, w0, w1, factor):
+    # PC/test_pair_counters/test_marked_npairs_3d.py:235-575 (asserted with exact ==)
+    pts, rb = _grid_points()
+    n = len(pts)
+    plain = ORC.npairs_3d(pts, pts, rb, period=np.array([1.0, 1.0, 1.0]))
+    if w1 is None:
+        w = np.ones(n) * w0
+    else:
+        w = np.vstack([np.ones(n) * w0, np.ones(n) * w1]).T
+    res = mo.marked_npairs_3d(pts, pts, rb, wid, period=np.array([1.0, 1.0, 1.0]), weights1=w, weights2=w,
+                              approx_cell1_size=[0.3, 0.3, 0.3])
+    assert np.all(res == factor * plain)
+
+
+def test_marked_deterministic(mo):
+    rng = np.random.default_rng(5)
+    s = rng.uniform(0, 100, (20000, 3))
+    w = rng.uniform(0, 1, 20000)
+    rb = np.logspace(-1, 1, 10)
+    a = mo.marked_npairs_3d(s, s, rb, 1, period=100, weights1=w, weights2=w)
+    b = mo.marked_npairs_3d(s, s, rb, 1, period=100, weights1=w, weights2=w)
+    assert np.array_equal(a, b)
