@@ -63,7 +63,7 @@ class Result(object):
 
 def run(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0, wfunc=0,
         want_counts=True, want_sums=False, rows1=False, rows2=False, want_evals=False,
-        also_counts=False, device=None):
+        also_counts=False, device=None, pair_events=None):
     """Enqueue one request on the current stream and return the output tensors.
 
     mocks : list of dicts with n1, n2, period (3,), pbc, cell (3,) (cell <= 0: default)
@@ -146,6 +146,8 @@ def run(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0, wf
     req.workspace = ws.data_ptr()
     req.workspace_bytes = int(nbytes)
     req.stream = torch.cuda.current_stream(device).cuda_stream
+    if pair_events is not None:
+        req.pair_events[0], req.pair_events[1] = pair_events
     fn = {E.HP_NPAIRS_3D: lib.hp_npairs_3d, E.HP_NPAIRS_PROJECTED: lib.hp_npairs_projected,
           E.HP_NPAIRS_XY_Z: lib.hp_npairs_xy_z, E.HP_MARKED_NPAIRS_3D: lib.hp_marked_npairs_3d}[kind]
     E.check(fn(ctypes.byref(req)))

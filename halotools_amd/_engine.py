@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_lib", "libhalopair.so")
+# HALOPAIR_LIB selects another build of the same ABI (A/B comparisons of kernel variants)
+LIB_PATH = os.environ.get("HALOPAIR_LIB") or os.path.join(_HERE, "_lib", "libhalopair.so")
 
 HP_NPAIRS_3D = 0
 HP_NPAIRS_PROJECTED = 1
@@ -26,7 +27,7 @@ HP_ALSO_COUNTS = 16
 EXPORTED_SYMBOLS = (
     "hp_last_error", "hp_abi_version", "hp_workspace_size", "hp_npairs_3d",
     "hp_npairs_projected", "hp_npairs_xy_z", "hp_marked_npairs_3d", "hp_marked_backward",
-    "hp_debug_bin_lookup",
+    "hp_debug_bin_lookup", "hp_debug_hilbert",
 )
 
 
@@ -67,6 +68,7 @@ class HpRequest(ctypes.Structure):
         ("workspace", ctypes.c_void_p),
         ("workspace_bytes", ctypes.c_size_t),
         ("stream", ctypes.c_void_p),
+        ("pair_events", ctypes.c_void_p * 2),
     ]
 
 
@@ -103,6 +105,8 @@ def library():
                                         ctypes.POINTER(ctypes.c_double), ctypes.c_int64,
                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
     lib.hp_debug_bin_lookup.restype = ctypes.c_int
+    lib.hp_debug_hilbert.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32]
+    lib.hp_debug_hilbert.restype = ctypes.c_uint32
     _LIB = lib
     return lib
 

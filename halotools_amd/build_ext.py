@@ -29,14 +29,15 @@ FLAGS = [
 ]
 
 
-def build(verbose=False, extra=None):
-    os.makedirs(os.path.dirname(OUT), exist_ok=True)
-    cmd = [HIPCC] + FLAGS + (extra or []) + [os.path.join(SRC, s) for s in SOURCES] + ["-o", OUT + ".tmp"]
+def build(verbose=False, extra=None, out=None):
+    out = out or OUT
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    cmd = [HIPCC] + FLAGS + (extra or []) + [os.path.join(SRC, s) for s in SOURCES] + ["-o", out + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(OUT + ".tmp", OUT)
-    return OUT
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

@@ -135,7 +135,7 @@ struct Plan {
     int64_t ngroups[2] = {0, 0};
     int64_t max_groups[2] = {0, 0};
     int64_t total_cells = 0, total_keys = 0;
-    int bits_row = 1, bits_morton = 1;
+    int bits_row = 1, bits_hilbert = 1;
     std::vector<MockGeom> geom;
     Table tr, tp;
     int Kr = 0, Kp = 0, nb_count = 0;
@@ -280,7 +280,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     P.total_keys = key_off;
     if (cell_off >= (int64_t)0xffffffff || key_off > (int64_t)0xffffffff) return fail("mesh too large for 32-bit keys");
     P.bits_row = bits_for((uint64_t)cell_off);
-    P.bits_morton = bits_for((uint64_t)key_off);
+    P.bits_hilbert = bits_for((uint64_t)key_off);
 
     // ---- workspace layout
     size_t off = 0;
@@ -298,13 +298,13 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     P.o_status = take(16);
     P.o_gm[0] = take((size_t)P.ngroups[0] * 4 + 4);
     if (P.rows2) P.o_gm[1] = take((size_t)(P.same ? P.ngroups[0] : P.ngroups[1]) * 4 + 4);
-    // sorted sides: pass A lanes = (0, Morton), stream = (s2, RowMajor);
-    // pass B lanes = (s2, Morton), stream = (0, RowMajor)
+    // sorted sides: pass A lanes = (0, Hilbert), stream = (s2, RowMajor);
+    // pass B lanes = (s2, Hilbert), stream = (0, RowMajor)
     const int s2 = P.same ? 0 : 1;
-    P.sorted[0][kMorton].needed = true;
+    P.sorted[0][kHilbert].needed = true;
     P.sorted[s2][kRowMajor].needed = true;
     if (P.rows2) {
-        P.sorted[s2][kMorton].needed = true;
+        P.sorted[s2][kHilbert].needed = true;
         P.sorted[0][kRowMajor].needed = true;
     }
     int64_t nmax = 1;
@@ -403,7 +403,7 @@ static int run(const hp_request* r, int kind) {
             if (!S.needed) continue;
             // with SAME_SAMPLES only side 0 exists; the geometry rows of side 0 describe it
             if (build_sorted_side(dgeom, P.M, sd, (Order)o, X[sd], Y[sd], Z[sd], P.marked ? W[sd] : nullptr, P.nw,
-                                  P.n[sd], P.total_cells, o == kRowMajor ? P.bits_row : P.bits_morton, sb,
+                                  P.n[sd], P.total_cells, o == kRowMajor ? P.bits_row : P.bits_hilbert, sb,
                                   side_ptrs(ws, S), status, s))
                 return fail("cell build launch failed");
         }
@@ -418,7 +418,7 @@ static int run(const hp_request* r, int kind) {
     L.geom = dgeom;
     L.group_mock = gm0;
     L.ngroups = P.ngroups[0];
-    L.lane = side_ptrs(ws, P.sorted[0][kMorton]);
+    L.lane = side_ptrs(ws, P.sorted[0][kHilbert]);
     L.strm = side_ptrs(ws, P.sorted[s2][kRowMajor]);
     L.nw = P.nw;
     L.br = P.tr.spec;
@@ -439,6 +439,7 @@ static int run(const hp_request* r, int kind) {
             break;
     }
     int rc = 0;
+    if (r->pair_events[0]) hipEventRecord(static_cast<hipEvent_t>(r->pair_events[0]), s);
     if (!(P.marked && !P.sums && !P.counts && !P.rows1)) {
         rc = launch_pairs(L, s);
         if (rc == -2) return fail("histogram does not fit in LDS (too many bins)");
@@ -455,7 +456,7 @@ static int run(const hp_request* r, int kind) {
         B.also_count = 0;
         B.group_mock = gm1;
         B.ngroups = P.ngroups[s2];
-        B.lane = side_ptrs(ws, P.sorted[s2][kMorton]);
+        B.lane = side_ptrs(ws, P.sorted[s2][kHilbert]);
         B.strm = side_ptrs(ws, P.sorted[0][kRowMajor]);
         B.rows = r->rows2;
         B.pcount = nullptr;
@@ -464,6 +465,7 @@ static int run(const hp_request* r, int kind) {
         // with SAME_SAMPLES the side-1 rows of the geometry equal the side-0 rows
         if (launch_pairs(B, s)) return fail("pair kernel (swapped) launch failed");
     }
+    if (r->pair_events[1]) hipEventRecord(static_cast<hipEvent_t>(r->pair_events[1]), s);
     uint64_t* red = reinterpret_cast<uint64_t*>(ws + P.o_red);
     if (P.counts) {
         if (launch_reduce_counts_s(L.pcount, P.Kr, kind == HP_NPAIRS_XY_Z ? P.Kp : 0, dgeom, P.M, 0,
@@ -527,5 +529,9 @@ int hp_debug_bin_lookup(const double* r2, int32_t K, const double* d, int64_t n,
     }
     return 0;
 }
+
+// Host-only diagnostic: Hilbert index of a lane-ordering cell (tests check the
+// curve is a bijection whose consecutive cells share a face).
+uint32_t hp_debug_hilbert(uint32_t x, uint32_t y, uint32_t z, int32_t bits) { return hilbert3(x, y, z, bits); }
 
 }  // extern "C"

@@ -8,10 +8,10 @@
 // deterministic.  Two key flavours exist:
 //   row-major  (ix*ny + iy)*nz + iz  -- stream side; the cells of one (ix, iy)
 //              column are contiguous, so a column's z-range is one segment;
-//   Morton     interleaved bits     -- lane side; 64 consecutive points form a
-//              spatially compact group.
+//   Hilbert    3-D Hilbert index    -- lane side; 64 consecutive points form a
+//              spatially compact group (the curve has no jumps).
 // A key of mock m is offset by that mock's first cell (row-major) or first
-// Morton code, so one sort handles a whole batch of realisations.
+// Hilbert index, so one sort handles a whole batch of realisations.
 #include <cstring>
 
 #include "hp_internal.h"
@@ -109,8 +109,15 @@ __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, 
     if (order == kRowMajor) {
         key = (uint32_t)(G.cell_off + ((int64_t)ix * G.ndiv[1] + iy) * G.ndiv[2] + iz);
     } else {
-        const uint32_t code = (part1by2((uint32_t)ix) << 2) | (part1by2((uint32_t)iy) << 1) | part1by2((uint32_t)iz);
-        key = (uint32_t)(G.key_off + code);
+        // lane order: a power-of-two grid of 2^mbits cells per dimension, so the
+        // Hilbert curve covers the box exactly (no passes through empty padding)
+        const int hb = G.mbits;
+        const int hn = 1 << hb;
+        bool dummy = false;
+        const int hx = cell_index(x[p], G.period[0] / hn, hn, dummy, G.period[0], 0);
+        const int hy = cell_index(y[p], G.period[1] / hn, hn, dummy, G.period[1], 0);
+        const int hz = cell_index(z[p], G.period[2] / hn, hn, dummy, G.period[2], 0);
+        key = (uint32_t)(G.key_off + hilbert3((uint32_t)hx, (uint32_t)hy, (uint32_t)hz, hb));
     }
     keys[p] = key;
     if (bad && status) atomicOr(status, 1);

@@ -10,7 +10,7 @@
 namespace hp {
 
 constexpr int kWave = 64;            // CDNA wavefront width
-constexpr int kGroup = 64;           // lane-side points per wave work item
+constexpr int kGroup = 64;           // lane-side points per wave work item (Hilbert order)
 constexpr int kTableMax = 512;       // bucket table entries per axis
 constexpr int kRadixBits = 8;        // digit width of the LSD counting sort
 constexpr int kRadixDigits = 1 << kRadixBits;
@@ -23,12 +23,12 @@ struct alignas(64) MockGeom {
     double cell[3];       // fine-cell edge per dimension
     int32_t ndiv[3];      // cells per dimension
     int32_t pbc;
-    int32_t mbits;        // bits per dimension of the Morton code
+    int32_t mbits;        // bits per dimension of the lane-ordering (Hilbert) grid
     int32_t pad;
     int64_t p_off[2];     // first point of this mock on each side
     int64_t n[2];         // points of this mock on each side
     int64_t cell_off;     // first global cell id of this mock
-    int64_t key_off;      // first global Morton key of this mock
+    int64_t key_off;      // first global Hilbert key of this mock
     int64_t group_off[2]; // first lane group of this mock, per lane side
     int64_t ngroups[2];
     float search[3];      // search half-widths (window), with safety margin
@@ -68,7 +68,7 @@ struct SortBufs {
     uint32_t* digit_totals;  // [digits]
 };
 
-enum Order : int { kMorton = 0, kRowMajor = 1 };
+enum Order : int { kHilbert = 0, kRowMajor = 1 };
 
 // Sorted copy of one side in one order.
 struct SortedSide {
@@ -100,7 +100,7 @@ struct PairLaunch {
     const MockGeom* geom;
     const int32_t* group_mock;
     int64_t ngroups;
-    SortedSide lane;       // Morton-sorted lane side
+    SortedSide lane;       // Hilbert-sorted lane side
     SortedSide strm;       // row-major sorted stream side
     int nw;
     BinSpec br;            // radial / rp bins
@@ -129,13 +129,39 @@ __device__ __forceinline__ int64_t mock_of_point(const MockGeom* __restrict__ g,
     return lo;
 }
 
-__device__ __forceinline__ uint32_t part1by2(uint32_t v) {
-    v &= 0x3ffu;
-    v = (v | (v << 16)) & 0x030000ffu;
-    v = (v | (v << 8)) & 0x0300f00fu;
-    v = (v | (v << 4)) & 0x030c30c3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
+// Index of cell (x, y, z) along a 3-D Hilbert curve over a 2^b grid, b <= 10.
+// Method: J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707,
+// 381 (2004) -- AxesToTranspose, then interleave the transposed bits.  Unlike
+// a Morton code the curve never jumps: consecutive cells share a face, so any
+// run of consecutive points stays compact.
+__host__ __device__ __forceinline__ uint32_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int b) {
+    if (b <= 0) return 0u;
+    uint32_t X[3] = {x, y, z};
+    const uint32_t M = 1u << (b - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < 3; i++) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];
+    X[2] ^= X[1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    X[0] ^= t;
+    X[1] ^= t;
+    X[2] ^= t;
+    uint32_t h = 0;
+    for (int k = b - 1; k >= 0; k--)
+        h = (h << 3) | (((X[0] >> k) & 1u) << 2) | (((X[1] >> k) & 1u) << 1) | ((X[2] >> k) & 1u);
+    return h;
 }
 
 }  // namespace hp

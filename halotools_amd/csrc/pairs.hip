@@ -5,7 +5,7 @@
 //
 // Work decomposition (MI355X-first, not the reference's cell loop):
 //   * one wavefront = one "group" of 64 lane points that are consecutive in the
-//     Morton-sorted copy of the lane side, hence spatially compact;
+//     Hilbert-sorted copy of the lane side, hence spatially compact;
 //   * the wave reduces its points' bounding box, walks the (x, y) columns of
 //     fine cells within the search radius of that box, and for each column
 //     visits one contiguous z-run of the row-major sorted stream side (split
@@ -129,11 +129,16 @@ __device__ __forceinline__ uint32_t bin_addr(double d, const LdsEntry* __restric
 
 __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb) {
     // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-    // contiguous run of (Morton-ordered) groups so its L2 sees one region.
+    // contiguous run of (Hilbert-ordered) groups so its L2 sees one region.
     const int64_t q = nb / 8, r = nb % 8;
     const int64_t x = bid % 8, k = bid / 8;
     return x * q + (x < r ? x : r) + k;
 }
+
+#ifndef HP_BATCH
+#define HP_BATCH 8
+#endif
+constexpr int kBatch = HP_BATCH;  // stream points per scalar-load wait
 
 struct PairArgs {
     PairLaunch L;
@@ -263,10 +268,13 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
     uint64_t bound = 0;
     bool flushed = false;
 
-    // one pair: (qx, qy, qz) is the stream point, j its sorted index
-    auto pair = [&](double qx, double qy, double qz, int j, double shx, double shy, double shz, int sxs, int sys,
-                    int szs, double ax, double ay, double az) __attribute__((always_inline)) {
-        double dx, dy, dz;
+    // Pairs are processed in batches: the stream points of a batch arrive with
+    // one scalar-load wait, all distances are formed, then all bucket-table
+    // reads are issued back to back (one LDS wait), then the fire-and-forget
+    // LDS atomics.  Source order keeps the table reads ahead of the atomics,
+    // which the compiler may not reorder because both touch LDS.
+    auto sep = [&](double qx, double qy, double qz, double shx, double shy, double shz, int sxs, int sys, int szs,
+                   double ax, double ay, double az, double& dx, double& dy, double& dz) __attribute__((always_inline)) {
         if (!SWAP) {
             dx = ax - qx;
             dy = ay - qy;
@@ -279,28 +287,29 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
             dy = qy - py;
             dz = qz - pz;
         }
-        if (MODE == kModeCount3D) {
-            const double dsq = dx * dx + dy * dy + dz * dz;
-            const uint32_t a = bin_addr<MULTI>(dsq, s_tr, shr, bshr, nbr1, step_r, gedges_r);
-            __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(hl + a), 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-        } else if (MODE == kModeProj) {
-            const double dxy = dx * dx + dy * dy;
-            const double dz2 = dz * dz;
-            uint32_t a = bin_addr<MULTI>(dxy, s_tr, shr, bshr, nbr1, step_r, gedges_r);
-            a = (dz2 <= pimax2) ? a : trash_r;
-            __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(hl + a), 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-        } else if (MODE == kModeXYZ) {
+    };
+    // histogram byte offset (within this wave's rows, before the lane offset)
+    auto addr_of = [&](double dx, double dy, double dz) __attribute__((always_inline)) -> uint32_t {
+        if (MODE == kModeProj) {
             const double dxy = dx * dx + dy * dy;
             const double dz2 = dz * dz;
             const uint32_t a = bin_addr<MULTI>(dxy, s_tr, shr, bshr, nbr1, step_r, gedges_r);
-            const uint32_t b = bin_addr<MULTI>(dz2, s_tp, shp, bshp, nbp1, 4u, gedges_p);
-            __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(hl + a + b), 1u, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            return (dz2 <= pimax2) ? a : trash_r;
+        } else if (MODE == kModeXYZ) {
+            const double dxy = dx * dx + dy * dy;
+            const double dz2 = dz * dz;
+            return bin_addr<MULTI>(dxy, s_tr, shr, bshr, nbr1, step_r, gedges_r) +
+                   bin_addr<MULTI>(dz2, s_tp, shp, bshp, nbp1, 4u, gedges_p);
         } else {
             const double dsq = dx * dx + dy * dy + dz * dz;
-            const uint32_t a = bin_addr<MULTI>(dsq, s_tr, shr, bshr, nbr1, step_r, gedges_r);
+            return bin_addr<MULTI>(dsq, s_tr, shr, bshr, nbr1, step_r, gedges_r);
+        }
+    };
+    auto accumulate = [&](uint32_t a, int j) __attribute__((always_inline)) {
+        if (!F64) {
+            __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(hl + a), 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
             double v;
             if (MODE == kModeJac) {
                 v = SW[j];
@@ -374,20 +383,31 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
                     if (sys) ay = py - shy;
                     if (szs) az = pz - shz;
                 }
+                constexpr int B = kBatch;
                 int j = jb;
-                for (; j + 4 <= je; j += 4) {
-                    double bx[4], by[4], bz[4];
+                for (; j + B <= je; j += B) {
+                    double bx[B], by[B], bz[B];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
+                    for (int u = 0; u < B; u++) {
                         bx[u] = SX[j + u];
                         by[u] = SY[j + u];
                         bz[u] = SZ[j + u];
                     }
+                    uint32_t a[B];
 #pragma unroll
-                    for (int u = 0; u < 4; u++)
-                        pair(bx[u], by[u], bz[u], j + u, shx, shy, shz, sxs, sys, szs, ax, ay, az);
+                    for (int u = 0; u < B; u++) {
+                        double dx, dy, dz;
+                        sep(bx[u], by[u], bz[u], shx, shy, shz, sxs, sys, szs, ax, ay, az, dx, dy, dz);
+                        a[u] = addr_of(dx, dy, dz);
+                    }
+#pragma unroll
+                    for (int u = 0; u < B; u++) accumulate(a[u], j + u);
                 }
-                for (; j < je; j++) pair(SX[j], SY[j], SZ[j], j, shx, shy, shz, sxs, sys, szs, ax, ay, az);
+                for (; j < je; j++) {
+                    double dx, dy, dz;
+                    sep(SX[j], SY[j], SZ[j], shx, shy, shz, sxs, sys, szs, ax, ay, az, dx, dy, dz);
+                    accumulate(addr_of(dx, dy, dz), j);
+                }
             }
         }
     }
@@ -479,7 +499,10 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
     if (L.br.nb > kTableMax || L.bp.nb > kTableMax) return -1;
     A.wave_bytes = align16(wave_bytes);
     const int static_lds = (int)sizeof(LdsEntry) * kTableMax * (L.mode == kModeXYZ ? 2 : 1);
-    int waves = 8;
+#ifndef HP_MAX_WAVES
+#define HP_MAX_WAVES 8
+#endif
+    int waves = HP_MAX_WAVES;
     while (waves > 1 && static_lds + waves * A.wave_bytes > 72 * 1024) waves >>= 1;
     A.smem = waves * A.wave_bytes;
     if (static_lds + A.smem > 160 * 1024) return -2;

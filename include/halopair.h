@@ -98,6 +98,8 @@ typedef struct hp_request {
     void* workspace;           /* device scratch                                       */
     size_t workspace_bytes;
     void* stream;              /* hipStream_t; NULL = default stream                   */
+    void* pair_events[2];      /* optional hipEvent_t pair recorded on `stream` right  */
+                               /* before and after the pair kernel(s) (profiling)      */
 } hp_request;
 
 /* Thread-local message describing the last error (empty string if none). */
@@ -139,6 +141,13 @@ int hp_marked_npairs_3d(const hp_request* req);
  * and grad are device arrays. */
 int hp_marked_backward(const double* rows, int32_t nr, const double* gsuffix,
                        const int64_t* offsets, int32_t nmocks, double* grad, void* stream);
+
+/* Diagnostics (host only, no GPU needed; used by the CPU test-suite). */
+/* The kernels' bucketed bin lookup: out[i] = number of suffix-minimum edges of r2 below d[i];
+ * info = {shift, base, table entries, multi}. */
+int hp_debug_bin_lookup(const double* r2, int32_t K, const double* d, int64_t n, int32_t* out, int32_t* info);
+/* Hilbert index of cell (x, y, z) on a 2^bits grid: the lane-group order of the pair kernels. */
+uint32_t hp_debug_hilbert(uint32_t x, uint32_t y, uint32_t z, int32_t bits);
 
 #ifdef __cplusplus
 }

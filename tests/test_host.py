@@ -44,7 +44,7 @@ def test_abi_struct_sizes():
     # hp_mock and hp_request must match the C layout of include/halopair.h
     assert ctypes.sizeof(E.HpMock) == 8 + 8 + 24 + 4 + 4 + 24
     off = E.HpRequest.workspace.offset
-    assert off % 8 == 0 and ctypes.sizeof(E.HpRequest) == E.HpRequest.stream.offset + 8
+    assert off % 8 == 0 and ctypes.sizeof(E.HpRequest) == E.HpRequest.pair_events.offset + 16
 
 
 @pytest.mark.parametrize("edges", [
@@ -170,3 +170,18 @@ def test_marked_weight_length_message():
     with pytest.raises(HalotoolsError) as err:
         marked_npairs_3d(s, s, np.array([0.1, 0.2]), 1, period=1.0, weights1=np.ones((50, 1, 1)))
     assert "You must either pass in a 1-D or 2-D array" in err.value.args[0]
+
+
+@pytest.mark.parametrize("bits", [1, 2, 3, 4])
+def test_hilbert_order_is_face_adjacent_bijection(bits):
+    lib = E.library()
+    n = 1 << bits
+    idx = {}
+    for x in range(n):
+        for y in range(n):
+            for z in range(n):
+                idx[lib.hp_debug_hilbert(x, y, z, bits)] = (x, y, z)
+    assert sorted(idx) == list(range(n ** 3))
+    for h in range(n ** 3 - 1):
+        a, b = np.array(idx[h]), np.array(idx[h + 1])
+        assert np.abs(a - b).sum() == 1, (h, idx[h], idx[h + 1])

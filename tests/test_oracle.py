@@ -1,0 +1,191 @@
+"""Pin the CPU oracle before trusting it (CPU only).
+
+The oracle (oracle/cpu_ref.c + oracle/mesh.py) restates the reference engines;
+here it is checked against
+  * the reference's closed-form known answers (PC/test_pair_counters/...),
+  * an independent brute force (oracle/brute.py, restating PC/pairs.py: it
+    takes min(|dx|, L-|dx|) and compares square roots, a different method),
+  * invariants: mesh independence, serial == threaded, symmetry, auto >= N,
+  * the reference mesh invariants (PC/test_pair_counters/test_rectangular_mesh.py),
+and each planted mistake (ref_set_mutation) must make at least one of these
+checks fail, which shows the checks have teeth.
+PC = halotools/mock_observables/pair_counters.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import brute
+from oracle import mesh as ORC
+from tests.helpers import (fixed_seed, generate_3d_regular_mesh, generate_locus_of_3d_points, grid_rbins,
+                           random_sample)
+
+
+@pytest.fixture(autouse=True)
+def _no_mutation():
+    ORC.set_mutation(0)
+    yield
+    ORC.set_mutation(0)
+
+
+# ----------------------------------------------------------- known answers
+def known_answer_failures():
+    """Run every closed-form check; return the list of failing check names."""
+    fails = []
+    d1 = generate_locus_of_3d_points(100, xc=0.1, yc=0.1, zc=0.1, seed=fixed_seed)
+    d2 = generate_locus_of_3d_points(100, xc=0.1, yc=0.1, zc=0.2, seed=fixed_seed)
+    if not np.all(ORC.npairs_3d(d1, d2, np.array((0.05, 0.15, 0.3)), period=1) == [0, 10000, 10000]):
+        fails.append("locus1")
+    d1 = generate_locus_of_3d_points(100, xc=0.1, yc=0.1, zc=0.05, seed=fixed_seed)
+    d2 = generate_locus_of_3d_points(100, xc=0.1, yc=0.1, zc=0.95, seed=fixed_seed)
+    if not np.all(ORC.npairs_3d(d1, d2, np.array((0.05, 0.15, 0.3)), period=1) == [0, 10000, 10000]):
+        fails.append("locus_pbc")
+    g = generate_3d_regular_mesh(10)
+    rb = grid_rbins()
+    if not np.all(ORC.npairs_3d(g, g, rb, period=1., approx_cell1_size=0.1) == [1000, 7000, 19000, 27000]):
+        fails.append("grid3d")
+    if not np.all(ORC.npairs_projected(g, g, rb, 0.001, period=1., approx_cell1_size=0.1) ==
+                  [1000, 5000, 9000, 9000]):
+        fails.append("grid_proj_small")
+    if not np.all(ORC.npairs_projected(g, g, rb, 0.11, period=1., approx_cell1_size=0.1) ==
+                  [3000, 15000, 27000, 27000]):
+        fails.append("grid_proj")
+    res = ORC.npairs_xy_z(g, g, rb, np.array([0, 0.11]), period=1., approx_cell1_size=0.1)
+    if not (np.all(res[:, 0] == [1000, 5000, 9000, 9000]) and np.all(res[:, 1] == [3000, 15000, 27000, 27000])):
+        fails.append("grid_xyz")
+    # marked, uniform weights (PC/test_pair_counters/test_marked_npairs_3d.py:235-261)
+    eps = 0.001
+    lg = np.linspace(0, 1 - eps, 10)
+    xx, yy, zz = np.array(np.meshgrid(lg, lg, lg))
+    pts = np.vstack([xx.flatten(), yy.flatten(), zz.flatten()]).T
+    rbm = np.array([0.0, 0.1, 0.2, 0.3])
+    plain = brute.npairs(pts, pts, rbm, period=1.0)
+    res = ORC.marked_npairs_3d(pts, pts, rbm, 1.0, np.ones(1000) * 3, np.ones(1000) * 3, 1)
+    if not np.all(res == 9.0 * plain):
+        fails.append("marked_uniform")
+    return fails
+
+
+def brute_force_failures():
+    fails = []
+    s = random_sample(600)
+    rb = np.array([0.001, 0.1, 0.2, 0.3])
+    for per in (1.0, None):
+        if not np.array_equal(ORC.npairs_3d(s, s, rb, period=per), brute.npairs(s, s, rb, period=per)):
+            fails.append("npairs_%s" % per)
+    s1, s2 = random_sample(200, seed=5), random_sample(150, seed=6)
+    rp = np.arange(0, 0.31, 0.1)
+    pi = np.arange(0, 0.31, 0.1)
+    for per in (1.0, None):
+        if not np.array_equal(ORC.npairs_xy_z(s1, s2, rp, pi, period=per),
+                              brute.xy_z_npairs(s1, s2, rp, pi, period=per)):
+            fails.append("xyz_%s" % per)
+        if not np.array_equal(ORC.npairs_projected(s1, s2, rp[1:], 0.1, period=per),
+                              brute.xy_z_npairs(s1, s2, rp[1:], [0, 0.1], period=per)[:, 1]):
+            fails.append("proj_%s" % per)
+    w = np.random.RandomState(fixed_seed).random_sample(600)
+    rbm = np.array([0.0, 0.1, 0.2, 0.3])
+    for per in (1.0, None):
+        a = ORC.marked_npairs_3d(s, s, rbm, per, w, w, 1)
+        b = brute.wnpairs(s, s, rbm, period=per, weights1=w, weights2=w)
+        if not np.allclose(a, b, rtol=1e-13, atol=0):
+            fails.append("marked_%s" % per)
+    return fails
+
+
+def test_known_answers():
+    assert known_answer_failures() == []
+
+
+def test_brute_force_agreement():
+    assert brute_force_failures() == []
+
+
+def test_marked_generic_wfuncs_vs_brute_force():
+    s = random_sample(300, seed=8)
+    rb = np.array([0.0, 0.1, 0.2, 0.3])
+    for wid, width in [(2, 1), (3, 2), (5, 2), (7, 2), (9, 2), (11, 2), (12, 4), (14, 3), (16, 5)]:
+        w = np.random.RandomState(wid).random_sample((300, width)) - 0.5
+        w[:, 0] = np.round(w[:, 0] * 4) / 4
+        a = ORC.marked_npairs_3d(s, s, rb, 1.0, w, w, wid)
+        b = brute.wnpairs_generic(s, s, rb, w, w, lambda u, v: ORC.wfunc(wid, u, v), period=1.0)
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("wid,a,b,expected", [
+    (1, [2.0], [3.0], 6.0), (2, [2.0], [3.0], 5.0), (3, [1.0, 2.0], [1.0, 3.0], 6.0), (3, [1.0, 2.0], [2.0, 3.0], 0.0),
+    (4, [1.0, 2.0], [2.0, 3.0], 6.0), (5, [1.0, 2.0], [2.0, 3.0], 6.0), (6, [1.0, 2.0], [2.0, 3.0], 0.0),
+    (7, [1.0, 0.5], [2.0, 3.0], 3.0), (8, [1.0, 0.5], [1.2, 3.0], 3.0), (9, [1.0, 0.5], [1.2, 3.0], 3.0),
+    (10, [1.0, 0.5], [1.2, 3.0], 0.0), (11, [1.0, 0.5], [0.6, 3.0], 3.0),
+    (12, [2.0, 1.0, 0.0, 0.0], [3.0, 0.5, 1.0, 1.0], 3.0), (13, [2.0, 1.0, 0.0, 0.0], [3.0, 0.5, 1.0, 1.0], 1.5),
+    (14, [2.0, 1.0, 1.0], [3.0, 0.5, 1.0], 9.0), (15, [2.0, 1.0, 1.0], [3.0, 0.5, 1.0], 13.5),
+    (16, [1.0, 1.0, 0.0, 0.0, 7.0], [1.0, 2.0, 0.0, 0.0, 7.0], 4.0),
+    (17, [1.0, 1.0, 0.0, 0.0, 7.0], [1.0, 2.0, 0.0, 0.0, 7.0], 0.0),
+])
+def test_marking_function_values(wid, a, b, expected):
+    # PC/marked_cpairs/marking_functions.pyx:14-217 evaluated by hand
+    assert ORC.wfunc(wid, np.array(a), np.array(b)) == expected
+
+
+# ----------------------------------------------------------- invariants
+def test_mesh_independence_and_threads():
+    s1, s2 = random_sample(1500, seed=3), random_sample(1200, seed=4)
+    rb = np.array([0.01, 0.05, 0.1, 0.2])
+    ref = ORC.npairs_3d(s1, s2, rb, period=1.0)
+    for a1, a2 in [(0.1, 0.1), (0.2, 0.05), ([0.3, 0.25, 0.21], [0.1, 0.15, 0.3]), (0.05, 0.3)]:
+        assert np.array_equal(ORC.npairs_3d(s1, s2, rb, period=1.0, approx_cell1_size=a1, approx_cell2_size=a2),
+                              ref)
+    for nt in (2, 3, 7):
+        assert np.array_equal(ORC.npairs_3d(s1, s2, rb, period=1.0, num_threads=nt), ref)
+
+
+def test_symmetry_and_self_pairs():
+    s1, s2 = random_sample(800, seed=13), random_sample(700, seed=14)
+    rb = np.array([0.05, 0.1, 0.2])
+    assert np.array_equal(ORC.npairs_3d(s1, s2, rb, period=1.0), ORC.npairs_3d(s2, s1, rb, period=1.0))
+    auto = ORC.npairs_3d(s1, s1, np.array([0.0, 0.1]), period=1.0)
+    assert auto[0] == len(s1) and auto[1] >= len(s1)
+
+
+def test_exact_and_fast_builds_agree():
+    s = random_sample(3000, seed=15)
+    rb = np.logspace(-2, np.log10(0.3), 12)
+    assert np.array_equal(ORC.npairs_3d(s, s, rb, period=1.0), ORC.npairs_3d(s, s, rb, period=1.0, fast=True))
+
+
+# ----------------------------------------------------------- reference mesh invariants
+def test_mesh_variations():
+    # PC/test_pair_counters/test_rectangular_mesh.py:88-135
+    npts1, npts2 = 90, 200
+    opts = itertools.product((1, 7), (1 / 15., 0.21, 0.95), (1 / 15., 1 / 5., 1. / np.pi), (1 / 15., 1 / 5., 1. / np.pi),
+                             (1 / 9., 1 / 20.), (True, False))
+    for period, zcm, a1, a2, search, pbc in opts:
+        p1 = generate_locus_of_3d_points(npts1, xc=0.1 * period, yc=0.1 * period, zc=0.1 * period, seed=fixed_seed)
+        p2 = generate_locus_of_3d_points(npts2, xc=0.1 * period, yc=0.1 * period, zc=zcm * period, seed=fixed_seed)
+        dm = ORC.RefDoubleMesh(p1[:, 0], p1[:, 1], p1[:, 2], p2[:, 0], p2[:, 1], p2[:, 2], [a1] * 3, [a2] * 3,
+                               [search] * 3, [period] * 3, pbc)
+        for m, n in ((dm.mesh1, npts1), (dm.mesh2, npts2)):
+            assert np.isclose(m.xcell_size * m.num_xdivs, m.xperiod)
+            assert m.ncells == m.num_xdivs * m.num_ydivs * m.num_zdivs
+            ci = m.cell_id_indices
+            assert np.all(ci <= n) and np.all(np.diff(ci) >= 0) and np.sum(np.diff(ci)) == n
+            assert len(ci) == m.ncells + 1
+        assert dm.mesh2.num_xdivs // dm.mesh1.num_xdivs * dm.mesh2.xcell_size == dm.mesh1.xcell_size
+        assert search <= dm.mesh1.xcell_size
+
+
+def test_sample1_cell_size_limit():
+    with pytest.raises(ValueError) as err:
+        ORC.sample1_cell_size(1, 0.5, 0.1)
+    assert "Input ``search_length`` cannot exceed period/3" in err.value.args[0]
+
+
+# ----------------------------------------------------------- planted mistakes
+@pytest.mark.parametrize("mutation", [1, 2, 3, 4])
+def test_planted_mistakes_are_caught(mutation):
+    """1: '<' for '<='; 2: periodic shift dropped; 3: window one cell short; 4: bins not squared."""
+    ORC.set_mutation(mutation)
+    fails = known_answer_failures() + brute_force_failures()
+    ORC.set_mutation(0)
+    assert fails, "mutation %d went unnoticed" % mutation
