@@ -145,6 +145,8 @@ struct Plan {
     Side sorted[2][2];  // [data side][order]
     size_t o_keys[2] = {0, 0}, o_vals[2] = {0, 0}, o_tiles = 0, o_digits = 0;
     size_t o_pcount = 0, o_psum = 0, o_pevals = 0, o_red = 0, o_status = 0;
+    bool elim = false;
+    size_t o_corr = 0, o_rows_tmp = 0;
     size_t total = 0;
     std::vector<unsigned char> consts;  // bytes [0, o_gm[0])
 };
@@ -336,6 +338,12 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     const int64_t red = reduce_scratch_elems(std::max<int64_t>(P.max_groups[0], 1), P.M,
                                              std::max(P.nb_count, P.Kr));
     P.o_red = take((size_t)red * 8);
+    // auto-count with shared weights: rows2 = rows1 + seam corrections, no swapped pass
+    P.elim = P.rows2 && P.same && P.marked && r->wfunc == 1;
+    if (P.elim) {
+        P.o_corr = take((size_t)std::max<int64_t>(P.n[0], 1) * (size_t)P.Kr * 8);
+        if (!P.rows1) P.o_rows_tmp = take((size_t)std::max<int64_t>(P.n[0], 1) * (size_t)P.Kr * 8);
+    }
     P.total = align_up(off, 256);
 
     // ---- constant blob (geometry, edges, tables) that goes to the workspace
@@ -438,14 +446,23 @@ static int run(const hp_request* r, int kind) {
             L.rows = P.rows1 ? r->rows1 : nullptr;
             break;
     }
+    double* corr = nullptr;
+    if (P.elim) {
+        corr = reinterpret_cast<double*>(ws + P.o_corr);
+        if (!P.rows1) L.rows = reinterpret_cast<double*>(ws + P.o_rows_tmp);
+        L.corr = corr;
+        launch_fill_u32(reinterpret_cast<uint32_t*>(corr), 0u, P.n[0] * (int64_t)P.Kr * 2, s);
+    }
     int rc = 0;
     if (r->pair_events[0]) hipEventRecord(static_cast<hipEvent_t>(r->pair_events[0]), s);
-    if (!(P.marked && !P.sums && !P.counts && !P.rows1)) {
+    if (!(P.marked && !P.sums && !P.counts && !P.rows1 && !P.elim)) {
         rc = launch_pairs(L, s);
         if (rc == -2) return fail("histogram does not fit in LDS (too many bins)");
         if (rc) return fail("pair kernel launch failed");
     }
-    if (P.rows2) {
+    if (P.elim) {
+        if (launch_add_rows(L.rows, corr, r->rows2, P.n[0] * (int64_t)P.Kr, s)) return fail("row add launch failed");
+    } else if (P.rows2) {
         // pass B: lanes are sample 2, the stream is sample 1 (same arithmetic,
         // (x1 - x2shift) - x2, with the shift applied to the sample-1 point)
         int32_t* gm1 = reinterpret_cast<int32_t*>(ws + P.o_gm[1]);

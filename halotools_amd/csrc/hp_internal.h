@@ -109,6 +109,7 @@ struct PairLaunch {
     uint64_t* pcount;      // [ngroups][nbins_count]
     double* psum;          // [ngroups][nr]
     double* rows;          // [n_lane_total][nr] (original order) or nullptr
+    double* corr;          // auto-counts: rows2 - rows1 corrections from seam pairs, or nullptr
     uint64_t* pevals;      // [ngroups]
 };
 enum Mode : int { kModeCount3D = 0, kModeProj = 1, kModeXYZ = 2, kModeWSum = 3, kModeJac = 4 };
@@ -117,6 +118,7 @@ int launch_pairs(const PairLaunch& L, hipStream_t s);
 // reduce.hip
 int launch_rows_dot(const double* rows, int nr, const double* gsuffix, const int64_t* offsets,
                     int nmocks, double* grad, hipStream_t s);
+int launch_add_rows(const double* a, const double* b, double* out, int64_t n, hipStream_t s);
 
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ int64_t mock_of_point(const MockGeom* __restrict__ g, int nmocks, int side,

@@ -385,6 +385,33 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
                 }
                 constexpr int B = kBatch;
                 int j = jb;
+                if (MODE == kModeJac && !SWAP && L.corr && (sxs || sys || szs)) {
+                    // Auto-count with one set of weights: the swapped orientation
+                    // of this pair, computed as (x2 + x2shift) - x1, is what the
+                    // sample-2 rows need.  Across the periodic seam the two
+                    // orientations may round apart; record the rare disagreement
+                    // as a correction so rows2 = rows1 + corr exactly.
+                    const uint32_t* __restrict__ SP = L.strm.perm;
+                    for (; j < je; j++) {
+                        const double qx = SX[j], qy = SY[j], qz = SZ[j];
+                        double dx, dy, dz;
+                        sep(qx, qy, qz, shx, shy, shz, sxs, sys, szs, ax, ay, az, dx, dy, dz);
+                        const uint32_t a = addr_of(dx, dy, dz);
+                        const double rx = sxs ? (qx + shx) - px : dx;
+                        const double ry = sys ? (qy + shy) - py : dy;
+                        const double rz = szs ? (qz + shz) - pz : dz;
+                        const uint32_t ar = addr_of(rx, ry, rz);
+                        accumulate(a, j);
+                        if (ar != a) {
+                            const uint32_t sf = a / step_r, sr = ar / step_r;
+                            double* crow = L.corr + (int64_t)SP[j] * Kr;
+                            if (sf < (uint32_t)Kr)
+                                __hip_atomic_fetch_add(crow + sf, lw[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            if (sr < (uint32_t)Kr)
+                                __hip_atomic_fetch_add(crow + sr, -lw[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        }
+                    }
+                }
                 for (; j + B <= je; j += B) {
                     double bx[B], by[B], bz[B];
 #pragma unroll

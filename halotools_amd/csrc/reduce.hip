@@ -175,4 +175,18 @@ int launch_rows_dot(const double* rows, int nr, const double* gsuffix, const int
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+__global__ void k_add_rows(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out,
+                           int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = a[i] + b[i];
+}
+
+int launch_add_rows(const double* a, const double* b, double* out, int64_t n, hipStream_t s) {
+    if (n <= 0) return 0;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_add_rows, dim3((unsigned)blocks), dim3(256), 0, s, a, b, out, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 }  // namespace hp

@@ -121,3 +121,51 @@ def test_backward_deterministic(ad, gpu):
         ad.marked_npairs_3d_product(s, s, rb, w, w, period=1.0).sum().backward()
         grads.append(w.grad.clone())
     assert torch.equal(grads[0], grads[1])
+
+
+def _seam_pair():
+    """Two points across the x seam of the unit box whose two pair orientations,
+    (x1 + L) - x2 and (x2 - L) - x1, square to different doubles, and an edge
+    r with r*r between them."""
+    rng = np.random.default_rng(0)
+    while True:
+        xi, xj = rng.uniform(0.0, 0.05), rng.uniform(0.95, 1.0)
+        df, dr = ((xi + 1.0) - xj) ** 2, ((xj - 1.0) - xi) ** 2
+        if df != dr:
+            break
+    lo, hi = min(df, dr), max(df, dr)
+    r = np.sqrt(lo)
+    for _ in range(100):
+        if lo <= r * r < hi:
+            break
+        r = np.nextafter(r, 0.0) if r * r >= hi else np.nextafter(r, 1.0)
+    assert lo <= r * r < hi
+    return xi, xj, r
+
+
+def test_seam_correction_matches_swapped_pass(gpu):
+    from halotools_amd import _device as D, _engine as E
+    xi, xj, r = _seam_pair()
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(0, 1, (600, 3))
+    pts[10] = [xi, 0.5, 0.5]
+    pts[20] = [xj, 0.5, 0.5]
+    w = rng.integers(1, 4, 600).astype(np.float64)  # small integers: every sum is exact
+    rb = np.array([0.5 * r, r, 0.2])
+    t = torch.tensor(pts.T.copy(), device=gpu)
+    cols = (t[0], t[1], t[2])
+    cols_copy = tuple(c.clone() for c in cols)
+    wt = torch.tensor(w, device=gpu).reshape(-1, 1)
+    mock = [dict(n1=600, n2=600, period=[1.0] * 3, pbc=True)]
+    elim = D.run(E.HP_MARKED_NPAIRS_3D, mock, cols, cols, rb * rb, same=True, w1=wt, nw=1, wfunc=1,
+                 want_counts=False, want_sums=True, rows1=True, rows2=True, device=gpu)
+    swapped = D.run(E.HP_MARKED_NPAIRS_3D, mock, cols, cols_copy, rb * rb, same=False, w1=wt, w2=wt.clone(), nw=1,
+                    wfunc=1, want_counts=False, want_sums=True, rows1=True, rows2=True, device=gpu)
+    torch.cuda.synchronize()
+    r1e, r2e = elim.rows1.cpu().numpy(), elim.rows2.cpu().numpy()
+    r1s, r2s = swapped.rows1.cpu().numpy(), swapped.rows2.cpu().numpy()
+    assert np.array_equal(r1e, r1s)
+    assert np.array_equal(r2e, r2s)
+    # the two special points really are binned differently by the two orientations
+    assert not np.array_equal(r1e[[10, 20]], r2e[[10, 20]])
+    assert np.array_equal(elim.sums.cpu().numpy(), swapped.sums.cpu().numpy())
