@@ -61,14 +61,31 @@ class Result(object):
     pass
 
 
-def run(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0, wfunc=0,
-        want_counts=True, want_sums=False, rows1=False, rows2=False, want_evals=False,
-        also_counts=False, device=None, pair_events=None):
-    """Enqueue one request on the current stream and return the output tensors.
+def run(*args, **kwargs):
+    """Enqueue one request on the current stream and return the output tensors (see ``prepare``)."""
+    res = prepare(*args, **kwargs)
+    launch(res)
+    return res
+
+
+def launch(res, stream=None):
+    """(Re)issue a prepared request on `stream` (default: the current stream of its device)."""
+    req = res.req
+    req.stream = stream if stream is not None else torch.cuda.current_stream(res.device).cuda_stream
+    E.check(res.fn(ctypes.byref(req)))
+
+
+def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0, wfunc=0,
+            want_counts=True, want_sums=False, rows1=False, rows2=False, want_evals=False,
+            also_counts=False, device=None, pair_events=None):
+    """Build a request with all of its output and scratch buffers allocated.
 
     mocks : list of dicts with n1, n2, period (3,), pbc, cell (3,) (cell <= 0: default)
     s1, s2 : tuples (x, y, z) of float64 device tensors, mocks concatenated
     r2 : host float64 array of squared radial / rp edges (already squared like the reference)
+    The returned object keeps every buffer the request points to alive; the
+    same request can be launched again (e.g. inside HIP-graph capture) after
+    its input buffers are refreshed in place.
     """
     device = torch.device(device) if device is not None else s1[0].device
     M = len(mocks)
@@ -145,21 +162,23 @@ def run(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0, wf
     ws = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
     req.workspace = ws.data_ptr()
     req.workspace_bytes = int(nbytes)
-    req.stream = torch.cuda.current_stream(device).cuda_stream
     if pair_events is not None:
         req.pair_events[0], req.pair_events[1] = pair_events
-    fn = {E.HP_NPAIRS_3D: lib.hp_npairs_3d, E.HP_NPAIRS_PROJECTED: lib.hp_npairs_projected,
-          E.HP_NPAIRS_XY_Z: lib.hp_npairs_xy_z, E.HP_MARKED_NPAIRS_3D: lib.hp_marked_npairs_3d}[kind]
-    E.check(fn(ctypes.byref(req)))
-    res.workspace = ws  # keep alive until the stream has consumed it (caching allocator)
+    res.fn = {E.HP_NPAIRS_3D: lib.hp_npairs_3d, E.HP_NPAIRS_PROJECTED: lib.hp_npairs_projected,
+              E.HP_NPAIRS_XY_Z: lib.hp_npairs_xy_z, E.HP_MARKED_NPAIRS_3D: lib.hp_marked_npairs_3d}[kind]
+    res.req = req
+    res.device = device
+    # keep every host and device buffer the request points to alive with it
+    res.keep = (marr, r2, p2, s1, s2, w1, w2)
+    res.workspace = ws
     res.cbuf = cbuf
     return res
 
 
-def backward_rows(rows, gsuffix, offsets):
+def backward_rows(rows, gsuffix, offsets, out=None):
     """grad[p] = rows[p] . gsuffix[mock(p)] on the current stream."""
     n = int(offsets[-1])
-    grad = torch.empty(n, dtype=torch.float64, device=rows.device)
+    grad = out if out is not None else torch.empty(n, dtype=torch.float64, device=rows.device)
     offs = (ctypes.c_int64 * len(offsets))(*[int(v) for v in offsets])
     g = gsuffix.contiguous()
     E.check(E.library().hp_marked_backward(

@@ -83,6 +83,7 @@ def marked_npairs_3d_product(sample1, sample2, rbins, weights1, weights2, period
     rbins = np.atleast_1d(rbins).astype("f8")
     rmax = np.max(rbins)
     rbins = P.process_bins(rbins, P.RBINS_MSG)
+    P.check_last_edge(rbins)
     if period is None:
         pbc = False
         x1, y1, z1, x2, y2, z2, period = P.enclose_in_box(

@@ -1,0 +1,176 @@
+"""Many mock realisations of uneven size counted in one launch per pass.
+
+The reference counts one sample at a time: every call builds its own mesh and
+runs its engine over it (PC/npairs_3d.py:111-150; PC =
+halotools/mock_observables/pair_counters), and model fitting calls it once per
+mock (halotools/empirical_models/factories/mock_factory_template.py:288-320).
+Here all mocks are concatenated: one counting sort builds every mock's cells
+(keys carry a per-mock offset), one pair-kernel launch covers the lane groups
+of all mocks (each group knows its mock's geometry), and the per-mock outputs
+come out of one fixed-order reduction.
+
+``HODBatchStep`` is the model-fitting step of the benchmark: for every mock,
+the npairs_3d auto-counts, the product-weighted marked sums with Zheng07
+central-occupation weights, and the gradient of a loss sum_k c_k W_k with
+respect to each mock's logMmin -- counts and marked Jacobian rows in one pair
+traversal, then the backward contraction -- all capturable in one HIP graph.
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _device as D
+from . import _engine as E
+from . import autotune
+from .autodiff import _MarkedProductSums, suffix_sums
+
+
+class MockBatch(object):
+    """Concatenated realisations sharing one periodic box size and one set of bins.
+
+    points : list of (N_m, 3) arrays/tensors, or one (sum N_m, 3) array with `sizes`
+    period : box length (scalar or 3-vector), the same for every mock
+    rbins  : radial bin edges (validated like npairs_3d's rbins)
+    """
+
+    def __init__(self, points, period, rbins, sizes=None, device=None, cells=None):
+        from .mock_observables.pair_counters import _process as P
+        self.device = D.require_gpu(device)
+        if sizes is None:
+            sizes = [len(p) for p in points]
+            if all(isinstance(p, torch.Tensor) for p in points):
+                allp = torch.cat([p.to(self.device, torch.float64) for p in points])
+            else:
+                allp = torch.from_numpy(np.concatenate([np.asarray(p, dtype=np.float64) for p in points]))
+        else:
+            allp = points if isinstance(points, torch.Tensor) else torch.from_numpy(np.asarray(points, np.float64))
+        allp = allp.to(self.device, torch.float64)
+        self.sizes = [int(n) for n in sizes]
+        self.M = len(self.sizes)
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.cols = (allp[:, 0].contiguous(), allp[:, 1].contiguous(), allp[:, 2].contiguous())
+        self.rbins = P.process_bins(rbins, P.RBINS_MSG)
+        self.period = P.process_period(period)
+        rmax = float(np.max(self.rbins))
+        P.check_search_lengths((rmax, rmax, rmax), tuple(self.period))
+        self.mocks = []
+        for m, n in enumerate(self.sizes):
+            if cells is not None:
+                cell = [float(cells[m])] * 3
+            else:
+                edge = autotune.lookup(E.HP_NPAIRS_3D, n, self.period, rmax)
+                cell = [edge] * 3 if edge else [0.0, 0.0, 0.0]
+            self.mocks.append(dict(n1=n, n2=n, period=list(self.period), pbc=True, cell=cell))
+        mock_id = np.repeat(np.arange(self.M), self.sizes)
+        self.mock_id = torch.from_numpy(mock_id).to(self.device)
+
+    def npairs_3d(self):
+        """(M, K) int64 auto-counts, one row per mock (npairs_3d(p_m, p_m, rbins, period))."""
+        res = D.run(E.HP_NPAIRS_3D, self.mocks, self.cols, self.cols, self.rbins * self.rbins, same=True,
+                    device=self.device)
+        self._check(res)
+        return res.counts
+
+    def marked_npairs_3d_product(self, weights):
+        """(M, K) product-weighted auto sums, differentiable in `weights` (one weight per point)."""
+        w = weights.to(self.device, torch.float64)
+        plan = dict(mocks=self.mocks, s1=self.cols, s2=self.cols, r2=self.rbins * self.rbins, device=self.device,
+                    same_weights=True, offsets1=list(self.offsets), offsets2=list(self.offsets))
+        return _MarkedProductSums.apply(w, w, plan)
+
+    @staticmethod
+    def _check(res):
+        if int(res.cbuf[-1].item()) & 0xFFFFFFFF:
+            raise ValueError("All points must be finite and lie inside the periodic box [0, period]")
+
+
+class HODBatchStep(object):
+    """Counts + marked forward + backward for every mock of a batch, graph-capturable.
+
+    Weights are Zheng07 central occupations w_i = 0.5 (1 + erf((log10 M_i - logMmin_m) / sigma)),
+    restated from halotools/empirical_models/occupation_models/zheng07_components.py:149-153,
+    with one logMmin per mock.  The loss of mock m is sum_k c_k W_mk; the step
+    produces counts (M, K), W (M, K), loss (M,) and dloss/dlogMmin (M,).
+    """
+
+    def __init__(self, batch, log_mass, logMmin, sigma, c):
+        self.b = batch
+        dev = batch.device
+        self.log_mass = log_mass.to(dev, torch.float64).contiguous()
+        self.logMmin = logMmin.to(dev, torch.float64).clone()
+        self.sigma = float(sigma)
+        self.c = c.to(dev, torch.float64)
+        n = int(batch.offsets[-1])
+        self.w = torch.empty((n, 1), dtype=torch.float64, device=dev)
+        self.grad_w1 = torch.empty(n, dtype=torch.float64, device=dev)
+        self.grad_w2 = torch.empty(n, dtype=torch.float64, device=dev)
+        self.grad_logMmin = torch.zeros(batch.M, dtype=torch.float64, device=dev)
+        self.G = suffix_sums(self.c).expand(batch.M, -1).contiguous()
+        self.res = D.prepare(E.HP_MARKED_NPAIRS_3D, batch.mocks, batch.cols, batch.cols,
+                             batch.rbins * batch.rbins, same=True, w1=self.w, nw=1, wfunc=1,
+                             want_counts=False, want_sums=True, rows1=True, rows2=True, also_counts=True,
+                             device=dev)
+        self.graph = None
+
+    def _body(self):
+        b = self.b
+        u = (self.log_mass - self.logMmin[b.mock_id]) / self.sigma
+        self.w[:, 0].copy_(0.5 * (1.0 + torch.erf(u)))
+        D.launch(self.res)
+        D.backward_rows(self.res.rows1, self.G, b.offsets, out=self.grad_w1)
+        D.backward_rows(self.res.rows2, self.G, b.offsets, out=self.grad_w2)
+        dw = -torch.exp(-u * u) / (math.sqrt(math.pi) * self.sigma)
+        self.grad_logMmin.zero_()
+        self.grad_logMmin.index_add_(0, b.mock_id, (self.grad_w1 + self.grad_w2) * dw)
+        self.loss = (self.res.sums * self.c).sum(dim=1)
+
+    def run(self):
+        """One eager step; returns (counts, W, loss, dloss/dlogMmin)."""
+        self._body()
+        return self.outputs()
+
+    def capture(self):
+        """Record the whole step into one HIP graph (replayed by ``replay``)."""
+        s = torch.cuda.Stream(device=self.b.device)
+        s.wait_stream(torch.cuda.current_stream(self.b.device))
+        with torch.cuda.stream(s):
+            self._body()  # warm-up outside capture
+        torch.cuda.current_stream(self.b.device).wait_stream(s)
+        torch.cuda.synchronize(self.b.device)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self._body()
+        return self
+
+    def replay(self):
+        self.graph.replay()
+        return self.outputs()
+
+    def outputs(self):
+        return self.res.counts, self.res.sums, self.loss, self.grad_logMmin
+
+    def status(self):
+        return int(self.res.cbuf[-1].item()) & 0xFFFFFFFF
+
+
+def one_at_a_time(points, log_mass, logMmin, sigma, rbins, c, period):
+    """The same work mock by mock through the drop-in API (the comparison path).
+
+    Returns lists of counts, W, loss and dloss/dlogMmin per mock.
+    """
+    from .autodiff import marked_npairs_3d_product
+    from .mock_observables import npairs_3d
+    from .mocks import zheng07_ncen
+    out = ([], [], [], [])
+    for m in range(len(points)):
+        p = points[m]
+        counts = npairs_3d(p, p, rbins, period=period)
+        th = logMmin[m].detach().clone().requires_grad_(True)
+        w = zheng07_ncen(log_mass[m], th, sigma)
+        W = marked_npairs_3d_product(p, p, rbins, w, w, period=period)
+        loss = (W * c).sum()
+        loss.backward()
+        for lst, v in zip(out, (counts, W.detach(), loss.detach(), th.grad)):
+            lst.append(v)
+    return out

@@ -153,6 +153,8 @@ struct Plan {
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+constexpr double kCellTarget = 8.0;  // default stream points per fine cell
+
 static int bits_for(uint64_t range) {
     int b = 1;
     while (b < 32 && ((uint64_t)1 << b) < range) b++;
@@ -229,13 +231,19 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
         double Lmax = 0.0;
         int maxdiv = 1;
         int64_t ncells = 1;
+        // default fine-cell edge: about kCellTarget stream points per cell, kept
+        // within [R/3, R]; finer cells tighten the pair windows, coarser cells
+        // cut the per-segment overhead of sparse samples
+        const double volume = hm.period[0] * hm.period[1] * hm.period[2];
+        const double density = volume > 0.0 ? (double)nm[1] / volume : 0.0;
+        const double natural = density > 0.0 ? cbrt(kCellTarget / density) : INFINITY;
         for (int d = 0; d < 3; d++) {
             const double L = hm.period[d];
             if (!(L > 0.0) || !(L < INFINITY)) return fail("period / box length must be positive and finite");
             const double R = (d < 2) ? Rxy : Rz;
             if (hm.pbc && R > L / 3.0 * (1.0 + 1e-12))
                 return fail("search length exceeds period/3 (the reference's mesh limit)");
-            double b = hm.cell[d] > 0.0 ? hm.cell[d] : 0.5 * R;
+            double b = hm.cell[d] > 0.0 ? hm.cell[d] : std::min(std::max(natural, R / 3.0), R);
             if (!(b > 0.0)) b = L;  // zero search radius
             double nd = floor(L / b);
             if (nd < 1.0) nd = 1.0;

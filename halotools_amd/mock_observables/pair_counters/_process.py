@@ -75,6 +75,21 @@ def process_bins(bins, msg):
     return bins
 
 
+def check_last_edge(bins):
+    """Reject the one bin layout whose counts depend on the reference's mesh.
+
+    The reference only checks monotonicity for more than two edges
+    (PC/npairs_3d.py:178-179) and sizes its search by max(bins); a pair is
+    counted in the last bin when dsq <= bins[-1]**2.  If bins[-1]**2 exceeds
+    max(bins)**2 (a descending two-edge array with a negative edge), pairs
+    beyond the search length would count only where its mesh happens to
+    reach, so the result is not defined by the inputs.
+    """
+    if bins[-1] * bins[-1] > np.max(bins) * np.max(bins):
+        raise ValueError("Input bins whose last edge squared exceeds max(bins)**2 are not supported: "
+                         "the reference's result for them depends on its mesh")
+
+
 RBINS_MSG = "Input ``rbins`` must be a monotonically increasing 1D array with at least two entries"
 RP_BINS_MSG = ("Input ``rp_bins`` must be a monotonically increasing 1D array "
                "with at least two entries")

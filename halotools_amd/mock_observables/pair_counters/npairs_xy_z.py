@@ -26,8 +26,10 @@ def npairs_xy_z(sample1, sample2, rp_bins, pi_bins, period=None,
     x1, y1, z1 = P.columns(sample1)
     x2, y2, z2 = P.columns(sample2)
     rp_bins = P.process_bins(rp_bins, P.RP_BINS_MSG)
+    P.check_last_edge(rp_bins)
     rp_max = np.max(rp_bins)
     pi_bins = P.process_bins(pi_bins, P.PI_BINS_MSG)
+    P.check_last_edge(pi_bins)
     pi_max = np.max(pi_bins)
     if period is None:
         pbc = False

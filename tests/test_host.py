@@ -185,3 +185,9 @@ def test_hilbert_order_is_face_adjacent_bijection(bits):
     for h in range(n ** 3 - 1):
         a, b = np.array(idx[h]), np.array(idx[h + 1])
         assert np.abs(a - b).sum() == 1, (h, idx[h], idx[h + 1])
+
+
+def test_mesh_dependent_bins_rejected():
+    d1, d2 = _loci()
+    with pytest.raises(ValueError):
+        npairs_3d(d1, d2, np.array([0.1, -0.2]), period=1)
