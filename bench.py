@@ -24,6 +24,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector, with FMA (non-FMA issue ceiling is half)
+HBM_PEAK_GBPS = 8000.0
+ALG_BUILD_BYTES = 76
 
 
 def reference_candidates(pts, rmax, L):
@@ -50,6 +52,187 @@ def reference_candidates(pts, rmax, L):
     return int((h * win).sum())
 
 
+def bench_batched(dev, steps):
+    """64 mocks of uneven size: npairs_3d auto + product-marked forward + backward.
+
+    N_m log-uniform in [5e4, 2e6], uniform in a periodic L = 1000 box, log10 M
+    uniform in [11, 15], Zheng07 central weights with logMmin_m ~ U[11.5, 12.5]
+    and sigma = 0.3; loss_m = sum_k c_k W_mk.  The batched step (one HIP graph:
+    weights, one pair traversal for counts + Jacobian rows, backward
+    contraction, per-mock parameter gradients) is timed against the same work
+    done mock by mock through the drop-in API.
+    """
+    import numpy as np
+    import torch
+
+    from halotools_amd import _engine as E
+    from halotools_amd import batch, mocks
+    rng = np.random.default_rng([43, 5])
+    M, L = 64, 1000.0
+    sizes = np.round(np.exp(rng.uniform(np.log(5e4), np.log(2e6), M))).astype(int)
+    rbins = mocks.config_rbins(5)
+    pts = [torch.tensor(rng.uniform(0, L, (n, 3)), device=dev) for n in sizes]
+    lm = [torch.tensor(rng.uniform(11, 15, n), device=dev) for n in sizes]
+    logMmin = torch.tensor(rng.uniform(11.5, 12.5, M), device=dev)
+    c = torch.tensor(rng.normal(size=len(rbins)), device=dev)
+
+    b = batch.MockBatch(pts, L, rbins, device=dev)
+    # autotune: cell size per density bucket and the workgroup shape, timed on
+    # the batch's own largest mock of each bucket in the fused counts+Jacobian mode
+    t0 = time.perf_counter()
+    tuned = b.tune(E.HP_MARKED_NPAIRS_3D, reps=1)
+    t_tune = time.perf_counter() - t0
+
+    # cell build at this scale (all 64 mocks in one counting sort): HIP events
+    build_ms = []
+    for _ in range(3):
+        ev = _events(torch)
+        b.npairs_3d(build_events=(ev[0].cuda_event, ev[1].cuda_event))
+        torch.cuda.synchronize()
+        build_ms.append(ev[0].elapsed_time(ev[1]))
+    build_s = float(np.median(build_ms)) * 1e-3
+
+    step = batch.HODBatchStep(b, torch.cat(lm), logMmin, 0.3, c)
+    step.run()
+    torch.cuda.synchronize()
+    assert step.status() == 0
+    step.capture()
+    step.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step.replay()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    counts, W, loss, grad = [t.clone() for t in step.outputs()]
+
+    # finite differences for every mock at once (mocks are independent)
+    h = 1e-4
+    base = step.logMmin.clone()
+    step.logMmin.copy_(base + h)
+    lp = step.run()[2].clone()
+    step.logMmin.copy_(base - h)
+    lmn = step.run()[2].clone()
+    step.logMmin.copy_(base)
+    fd = (lp - lmn) / (2 * h)
+    fd_rel = float(((fd - grad).abs() / fd.abs()).max())
+
+    # the same work one mock at a time through the drop-in API
+    batch.one_at_a_time(pts[:2], lm[:2], logMmin[:2], 0.3, rbins, c, L)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref = batch.one_at_a_time(pts, lm, logMmin, 0.3, rbins, c, L)
+    torch.cuda.synchronize()
+    t_single = time.perf_counter() - t0
+    counts_equal = all(bool(torch.equal(counts[m], ref[0][m])) for m in range(M))
+    grad_single = torch.stack([g for g in ref[3]])
+    grad_rel = float(((grad_single - grad).abs() / grad_single.abs()).max())
+    npts = int(sizes.sum())
+    return {"mocks": M, "points": npts, "ms_per_mock": 1e3 * t_batch / M,
+            "batched_step_ms": 1e3 * t_batch, "one_at_a_time_ms_per_mock": 1e3 * t_single / M,
+            "speedup_vs_one_at_a_time": t_single / t_batch, "counts_equal_one_at_a_time": counts_equal,
+            "grad_max_rel_diff_vs_one_at_a_time": grad_rel, "grad_max_rel_diff_vs_central_fd": fd_rel,
+            "fd_step": h, "graph": "one HIP graph per step (torch.cuda.CUDAGraph around the library calls)",
+            "autotune": {"seconds": t_tune, "tile_waves": b.tile_waves,
+                         "buckets": {k: {"target_points_per_cell": v[0][0], "waves": v[0][1]}
+                                     for k, v in tuned.items()}},
+            "cell_build": _build_rate(npts, build_s)}
+
+
+def bench_other_configs(dev, steps):
+    """GPU-only timings of the other hot-path rows on their full-size inputs.
+
+    2M uniform points in L = 1000: npairs_xy_z with 16 log rp edges 0.1-30 and
+    pi edges 0..40 step 1, and npairs_projected with pi_max = 40.  1M uniform
+    halos in L = 1000 with log10 M ~ U[11, 15]: product-weighted marked sums
+    with Zheng07 central weights (logMmin = 12, sigma = 0.3), forward plus
+    backward to (logMmin, sigma).  Inputs are resident in HBM; each entry is the
+    mean wall time of whole calls and, for the projected pair kernels, their
+    own pair evaluations per second from HIP events.
+    """
+    import numpy as np
+    import torch
+
+    from halotools_amd import _device as D
+    from halotools_amd import _engine as E
+    from halotools_amd import mocks
+    from halotools_amd.autodiff import marked_npairs_3d_product
+    from halotools_amd.mock_observables import npairs_projected, npairs_xy_z
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            out = fn()
+        torch.cuda.synchronize()
+        return out, (time.perf_counter() - t0) / steps
+
+    out = {}
+    p3, _, L = mocks.config_points(3)
+    t3 = torch.from_numpy(np.ascontiguousarray(p3)).to(dev)
+    rp, pi = mocks.config_rbins(3), np.arange(0, 41, 1.0)
+    c_xyz, s_xyz = timed(lambda: npairs_xy_z(t3, t3, rp, pi, period=L))
+    c_prj, s_prj = timed(lambda: npairs_projected(t3, t3, rp, 40.0, period=L))
+    cols = tuple(t3[:, i].contiguous() for i in range(3))
+    mock = dict(n1=len(p3), n2=len(p3), period=[L] * 3, pbc=True)
+    for name, kind, p2, secs, cnt in (("npairs_xy_z", E.HP_NPAIRS_XY_Z, pi * pi, s_xyz, c_xyz),
+                                      ("npairs_projected", E.HP_NPAIRS_PROJECTED, np.array([1600.0]), s_prj,
+                                       c_prj)):
+        ev = _events(torch)
+        res = D.run(kind, [mock], cols, cols, rp * rp, p2=p2, same=True, want_evals=True, device=dev,
+                    pair_events=(ev[0].cuda_event, ev[1].cuda_event))
+        torch.cuda.synchronize()
+        assert torch.equal(res.counts[0], cnt)
+        ks = ev[0].elapsed_time(ev[1]) * 1e-3
+        evals = int(res.evals[0].item())
+        out[name] = {"workload": "2M uniform auto, L=1000, 16 log rp edges 0.1-30, "
+                                 + ("pi edges 0..40 step 1" if kind == E.HP_NPAIRS_XY_Z else "pi_max 40"),
+                     "ms_per_call": secs * 1e3, "pair_kernel_ms": ks * 1e3, "kernel_evals": evals,
+                     "kernel_evals_per_s": evals / ks, "fp64_peak_frac": 8.0 * evals / ks / 1e12 / FP64_PEAK_TFLOPS}
+    del t3, cols
+
+    rng = np.random.default_rng([43, 4])
+    p4 = mocks.uniform_box(1000000, 1000.0, rng)
+    logm = torch.tensor(rng.uniform(11, 15, len(p4)), device=dev)
+    t4 = torch.from_numpy(p4).to(dev)
+    rb = mocks.config_rbins(4)
+    c = torch.tensor(np.random.default_rng(7).normal(size=len(rb)), device=dev)
+    theta = torch.tensor([12.0, 0.3], dtype=torch.float64, device=dev, requires_grad=True)
+
+    def fwd_bwd():
+        theta.grad = None
+        w = mocks.zheng07_ncen(logm, theta[0], theta[1])
+        W = marked_npairs_3d_product(t4, t4, rb, w, w, period=1000.0)
+        (W * c).sum().backward()
+        return theta.grad
+
+    _, s_fb = timed(fwd_bwd)
+    out["marked_npairs_3d_product_fwd_bwd"] = {
+        "workload": "1M uniform halos, L=1000, 20 log edges 0.1-50, Zheng07 weights, d/d(logMmin, sigma)",
+        "ms_per_fwd_bwd": s_fb * 1e3}
+    return out
+
+
+def _events(torch):
+    ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+    for e in ev:
+        e.record()  # materialise the HIP event handles
+    return ev
+
+
+def _build_rate(npts, seconds):
+    """Cell build of one auto-count sample against the HBM roofline.
+
+    Algorithmic bytes of a counting-sort cell list: read x, y, z twice (48 B),
+    write the sorted x, y, z (24 B) and the int32 permutation (4 B): 76 B per
+    point; the cell-start table is negligible.
+    """
+    gbs = ALG_BUILD_BYTES * npts / seconds / 1e9
+    return {"points": npts, "ms": seconds * 1e3, "alg_bytes_per_point": ALG_BUILD_BYTES, "achieved_GBps": gbs,
+            "peak_GBps": HBM_PEAK_GBPS, "frac": gbs / HBM_PEAK_GBPS}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,6 +240,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host threads, capped at 16")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-mock batched leg")
+    ap.add_argument("--no-others", action="store_true", help="skip the xy_z / projected / marked timings")
     return ap.parse_args()
 
 
@@ -102,18 +287,21 @@ def main():
     cols = (t[:, 0].contiguous(), t[:, 1].contiguous(), t[:, 2].contiguous())
     mock = dict(n1=n, n2=n, period=[L, L, L], pbc=True)
     kern_ms = []
+    build_ms = []
     own_evals = None
     for _ in range(max(3, args.steps // 2)):
-        ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-        for e in ev:
-            e.record()  # materialise the HIP event handles
+        ev = _events(torch)
+        bev = _events(torch)
         res = D.run(E.HP_NPAIRS_3D, [mock], cols, cols, rbins * rbins, same=True, want_evals=True, device=dev,
-                    pair_events=(ev[0].cuda_event, ev[1].cuda_event))
+                    pair_events=(ev[0].cuda_event, ev[1].cuda_event),
+                    build_events=(bev[0].cuda_event, bev[1].cuda_event))
         torch.cuda.synchronize()
         kern_ms.append(ev[0].elapsed_time(ev[1]))
+        build_ms.append(bev[0].elapsed_time(bev[1]))
         own_evals = int(res.evals[0].item())
         assert np.array_equal(res.counts[0].cpu().numpy(), counts)
     kern_s = float(np.mean(kern_ms[1:])) * 1e-3 if len(kern_ms) > 1 else kern_ms[0] * 1e-3
+    build_s = float(np.mean(build_ms[1:])) * 1e-3 if len(build_ms) > 1 else build_ms[0] * 1e-3
     achieved = 8.0 * own_evals / kern_s / 1e12
 
     # -------- reference-engine work for this input (its own mesh), and the CPU baseline
@@ -132,6 +320,9 @@ def main():
                "sample": "the full workload once: reference mesh + engine loop restated in C (oracle/cpu_ref.c, "
                          "-Ofast, no -march), cell-1 range split over %d threads; %.2f s wall; counts checked "
                          "equal to the GPU's" % (threads, c1 - c0)}
+
+    batched = None if args.no_batched else bench_batched(dev, max(3, args.steps // 4))
+    others = None if args.no_others else bench_other_configs(dev, max(3, args.steps // 4))
 
     line = {
         "metric": "pair evals/s & %FP64 peak (npairs_3d, 1M pts); batched fwd+bwd ms per mock",
@@ -155,10 +346,13 @@ def main():
         "inrange_pairs_per_s": float(counts[-1]) / sec_per_step,
         "pair_kernel_ms": kern_s * 1e3,
         "fp64_peak_frac": achieved / FP64_PEAK_TFLOPS,
+        "cell_build": _build_rate(n, build_s),
         # compute-bound: FP64 vector ALU (a distance threshold, not a contraction, so no MFMA);
         # MI355X's FP64 peak is 78.6 TFLOP/s for vector and matrix alike
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None},
+        "batched_64_mocks": batched,
+        "other_configs": others,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
     }

@@ -77,12 +77,15 @@ def launch(res, stream=None):
 
 def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0, wfunc=0,
             want_counts=True, want_sums=False, rows1=False, rows2=False, want_evals=False,
-            also_counts=False, device=None, pair_events=None):
+            also_counts=False, device=None, pair_events=None, build_events=None, tile_waves=0):
     """Build a request with all of its output and scratch buffers allocated.
 
     mocks : list of dicts with n1, n2, period (3,), pbc, cell (3,) (cell <= 0: default)
     s1, s2 : tuples (x, y, z) of float64 device tensors, mocks concatenated
     r2 : host float64 array of squared radial / rp edges (already squared like the reference)
+    pair_events, build_events : optional (start, end) HIP event handles recorded
+        around the pair kernel(s) and around the cell build
+    tile_waves : wavefronts per pair-kernel workgroup (1, 2, 4, 8; 0 = library default)
     The returned object keeps every buffer the request points to alive; the
     same request can be launched again (e.g. inside HIP-graph capture) after
     its input buffers are refreshed in place.
@@ -164,6 +167,9 @@ def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0
     req.workspace_bytes = int(nbytes)
     if pair_events is not None:
         req.pair_events[0], req.pair_events[1] = pair_events
+    if build_events is not None:
+        req.build_events[0], req.build_events[1] = build_events
+    req.tile_waves = int(tile_waves)
     res.fn = {E.HP_NPAIRS_3D: lib.hp_npairs_3d, E.HP_NPAIRS_PROJECTED: lib.hp_npairs_projected,
               E.HP_NPAIRS_XY_Z: lib.hp_npairs_xy_z, E.HP_MARKED_NPAIRS_3D: lib.hp_marked_npairs_3d}[kind]
     res.req = req

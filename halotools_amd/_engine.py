@@ -55,7 +55,7 @@ class HpRequest(ctypes.Structure):
         ("nw", ctypes.c_int32),
         ("nr", ctypes.c_int32),
         ("np", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("tile_waves", ctypes.c_int32),
         ("r2", ctypes.POINTER(ctypes.c_double)),
         ("p2", ctypes.POINTER(ctypes.c_double)),
         ("search", ctypes.c_double * 3),
@@ -69,6 +69,7 @@ class HpRequest(ctypes.Structure):
         ("workspace_bytes", ctypes.c_size_t),
         ("stream", ctypes.c_void_p),
         ("pair_events", ctypes.c_void_p * 2),
+        ("build_events", ctypes.c_void_p * 2),
     ]
 
 

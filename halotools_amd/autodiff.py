@@ -45,7 +45,7 @@ class _MarkedProductSums(torch.autograd.Function):
                     w1=w1.detach().reshape(-1, 1).contiguous(),
                     w2=None if same else w2.detach().reshape(-1, 1).contiguous(), nw=1, wfunc=1,
                     want_counts=False, want_sums=True, rows1=True, rows2=(need2 and not same) or same,
-                    device=plan["device"])
+                    device=plan["device"], tile_waves=plan.get("tile_waves", 0))
         if int(res.cbuf[-1].item()) & 0xFFFFFFFF:
             raise ValueError("All points must be finite and lie inside the periodic box [0, period]")
         ctx.rows1 = res.rows1

@@ -1,13 +1,19 @@
-"""Cell-size autotuner for the pair kernels.
+"""Autotuner for the pair kernels: fine-cell size and workgroup tile shape.
 
-The GPU mesh has one free parameter per realisation: the edge of the fine
-stream-side cells.  Finer cells tighten each group's window (fewer candidate
-pairs), coarser cells cut the per-segment overhead.  The best edge depends on
-the point density, the search radius and the kernel mode, so the tuner times
-the count kernel on the caller's own points for a few targets of "stream points
-per cell" and keeps the fastest, keyed by (mode, log2 density bucket, search
-radius / box bucket).  Results are cached in memory and, when
-HALOTOOLS_AMD_TUNE_CACHE names a file, in that JSON file.
+The GPU mesh has one free parameter per realisation, the edge of the fine
+stream-side cells: finer cells tighten each lane group's window (fewer
+candidate pairs), coarser cells cut the per-segment overhead.  The pair kernel
+has one more, the tile shape: how many 64-point lane groups (wavefronts) share
+a workgroup, which trades LDS per workgroup against waves in flight per CU.
+The best choice depends on the point density, the search radius and the kernel
+mode, so the tuner times the caller's own points for a grid of "stream points
+per cell" targets and workgroup sizes and keeps the fastest, keyed by (mode,
+log2 density bucket, search radius / box bucket).  Results are cached in
+memory and, when HALOTOOLS_AMD_TUNE_CACHE names a file, in that JSON file.
+
+Modes: ``E.HP_NPAIRS_3D`` times npairs_3d auto-counts;
+``E.HP_MARKED_NPAIRS_3D`` times the product-weighted pass that also produces
+counts and both samples' Jacobian rows (the batched model-fitting step).
 """
 import json
 import math
@@ -21,6 +27,7 @@ from . import _device as D
 from . import _engine as E
 
 TARGETS = (2.0, 4.0, 8.0, 16.0, 32.0)
+WAVES = (2, 4, 8)
 _CACHE = {}
 
 
@@ -57,18 +64,35 @@ def cell_edge(target, n, period, rmax):
 
 
 def lookup(kind, n, period, rmax):
-    """Tuned cell edge for this shape of problem, or None (library default)."""
+    """Tuned (cell edge, workgroup waves) for this shape of problem, or None (library defaults)."""
     _load()
     t = _CACHE.get(key(kind, n, period, rmax))
-    return None if t is None else cell_edge(t, n, period, rmax)
+    if t is None:
+        return None
+    return cell_edge(float(t["target"]), n, period, rmax), int(t["waves"])
 
 
-def tune(points, period, rbins, reps=2, targets=TARGETS, device=None):
-    """Time npairs_3d auto-counts of `points` for each target and cache the fastest.
+def _request(kind, n, period, cols, rbins, edge, waves, device):
+    mock = dict(n1=n, n2=n, period=list(period), pbc=True, cell=[edge] * 3)
+    r2 = rbins * rbins
+    if kind == E.HP_NPAIRS_3D:
+        return D.prepare(kind, [mock], cols, cols, r2, same=True, device=device, tile_waves=waves)
+    w = torch.ones((n, 1), dtype=torch.float64, device=device)
+    return D.prepare(kind, [mock], cols, cols, r2, same=True, w1=w, nw=1, wfunc=1, want_counts=False,
+                     want_sums=True, rows1=True, rows2=True, also_counts=True, device=device, tile_waves=waves)
+
+
+def tune(points, period, rbins, kind=E.HP_NPAIRS_3D, reps=2, targets=TARGETS, waves=WAVES, device=None):
+    """Time auto-counts of `points` for every (target, waves) and cache the fastest.
 
     points : (N, 3) array or CUDA tensor inside the periodic box `period`.
-    Returns (best_target, {target: seconds}).
+    kind   : E.HP_NPAIRS_3D or E.HP_MARKED_NPAIRS_3D (see the module docstring).
+    Returns ((best_target, best_waves), {(target, waves): seconds}).  The counts
+    of every configuration are checked equal: the cell size and the tile shape
+    change the work schedule, never the result.
     """
+    if kind not in (E.HP_NPAIRS_3D, E.HP_MARKED_NPAIRS_3D):
+        raise ValueError("tune: kind must be HP_NPAIRS_3D or HP_MARKED_NPAIRS_3D")
     device = D.require_gpu(device)
     t = points if isinstance(points, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(points))
     t = t.to(device=device, dtype=torch.float64)
@@ -83,22 +107,24 @@ def tune(points, period, rbins, reps=2, targets=TARGETS, device=None):
     ref = None
     for target in targets:
         edge = cell_edge(target, n, period, rmax)
-        mock = dict(n1=n, n2=n, period=list(period), pbc=True, cell=[edge] * 3)
-        best = float("inf")
-        for _ in range(reps + 1):
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            res = D.run(E.HP_NPAIRS_3D, [mock], cols, cols, rbins * rbins, same=True, device=device)
-            torch.cuda.synchronize()
-            best = min(best, time.perf_counter() - t0)
-        counts = res.counts[0].cpu().numpy()
-        if ref is None:
-            ref = counts
-        elif not np.array_equal(ref, counts):
-            raise RuntimeError("counts changed with the cell size; this is a bug")
-        timing[target] = best
+        for wv in waves:
+            res = _request(kind, n, period, cols, rbins, edge, wv, device)
+            best = float("inf")
+            for _ in range(reps + 1):
+                torch.cuda.synchronize(device)
+                t0 = time.perf_counter()
+                D.launch(res)
+                torch.cuda.synchronize(device)
+                best = min(best, time.perf_counter() - t0)
+            counts = res.counts[0].cpu().numpy()
+            if ref is None:
+                ref = counts
+            elif not np.array_equal(ref, counts):
+                raise RuntimeError("counts changed with the cell size or tile shape; this is a bug")
+            timing[(target, wv)] = best
+            del res
     winner = min(timing, key=timing.get)
     _load()
-    _CACHE[key(E.HP_NPAIRS_3D, n, period, rmax)] = winner
+    _CACHE[key(kind, n, period, rmax)] = {"target": winner[0], "waves": winner[1]}
     _save()
     return winner, timing

@@ -34,7 +34,7 @@ class MockBatch(object):
     rbins  : radial bin edges (validated like npairs_3d's rbins)
     """
 
-    def __init__(self, points, period, rbins, sizes=None, device=None, cells=None):
+    def __init__(self, points, period, rbins, sizes=None, device=None, cells=None, tile_waves=0):
         from .mock_observables.pair_counters import _process as P
         self.device = D.require_gpu(device)
         if sizes is None:
@@ -54,21 +54,57 @@ class MockBatch(object):
         self.period = P.process_period(period)
         rmax = float(np.max(self.rbins))
         P.check_search_lengths((rmax, rmax, rmax), tuple(self.period))
-        self.mocks = []
-        for m, n in enumerate(self.sizes):
-            if cells is not None:
-                cell = [float(cells[m])] * 3
-            else:
-                edge = autotune.lookup(E.HP_NPAIRS_3D, n, self.period, rmax)
-                cell = [edge] * 3 if edge else [0.0, 0.0, 0.0]
-            self.mocks.append(dict(n1=n, n2=n, period=list(self.period), pbc=True, cell=cell))
+        self.rmax = rmax
+        self.tile_waves = int(tile_waves)
+        self.mocks = [dict(n1=n, n2=n, period=list(self.period), pbc=True, cell=[0.0] * 3) for n in self.sizes]
+        if cells is not None:
+            for m in range(self.M):
+                self.mocks[m]["cell"] = [float(cells[m])] * 3
+        else:
+            self._apply_tuning(E.HP_NPAIRS_3D, keep_waves=bool(tile_waves))
         mock_id = np.repeat(np.arange(self.M), self.sizes)
         self.mock_id = torch.from_numpy(mock_id).to(self.device)
 
-    def npairs_3d(self):
-        """(M, K) int64 auto-counts, one row per mock (npairs_3d(p_m, p_m, rbins, period))."""
+    def _apply_tuning(self, kind, keep_waves=False):
+        # per-mock cell edges from the tuner's cache; the workgroup shape (one
+        # per launch) from the entry of the mock with the most pair work
+        heaviest = int(np.argmax(self.sizes)) if self.M else 0
+        for m, n in enumerate(self.sizes):
+            hit = autotune.lookup(kind, n, self.period, self.rmax)
+            if hit is not None:
+                self.mocks[m]["cell"] = [hit[0]] * 3
+                if m == heaviest and not keep_waves:
+                    self.tile_waves = hit[1]
+
+    def tune(self, kind=E.HP_MARKED_NPAIRS_3D, **kw):
+        """Autotune cell edges and the tile shape on this batch's own mocks.
+
+        One representative (the largest mock) per density bucket of the tuner's
+        cache key is timed; every mock of the bucket then uses its result.
+        Returns {bucket key: ((target, waves), timings)}.
+        """
+        x, y, z = self.cols
+        done = {}
+        for m in np.argsort(self.sizes)[::-1]:
+            n = self.sizes[m]
+            k = autotune.key(kind, n, self.period, self.rmax)
+            if k in done or n < 64:
+                continue
+            a, b = int(self.offsets[m]), int(self.offsets[m + 1])
+            pts = torch.stack((x[a:b], y[a:b], z[a:b]), dim=1)
+            done[k] = autotune.tune(pts, self.period, self.rbins, kind=kind, device=self.device, **kw)
+        self._apply_tuning(kind)
+        return done
+
+    def npairs_3d(self, build_events=None, pair_events=None):
+        """(M, K) int64 auto-counts, one row per mock (npairs_3d(p_m, p_m, rbins, period)).
+
+        build_events / pair_events: optional (start, end) HIP event handles
+        recorded around the cell build and the pair kernel (profiling).
+        """
         res = D.run(E.HP_NPAIRS_3D, self.mocks, self.cols, self.cols, self.rbins * self.rbins, same=True,
-                    device=self.device)
+                    device=self.device, tile_waves=self.tile_waves, build_events=build_events,
+                    pair_events=pair_events)
         self._check(res)
         return res.counts
 
@@ -76,7 +112,8 @@ class MockBatch(object):
         """(M, K) product-weighted auto sums, differentiable in `weights` (one weight per point)."""
         w = weights.to(self.device, torch.float64)
         plan = dict(mocks=self.mocks, s1=self.cols, s2=self.cols, r2=self.rbins * self.rbins, device=self.device,
-                    same_weights=True, offsets1=list(self.offsets), offsets2=list(self.offsets))
+                    same_weights=True, offsets1=list(self.offsets), offsets2=list(self.offsets),
+                    tile_waves=self.tile_waves)
         return _MarkedProductSums.apply(w, w, plan)
 
     @staticmethod
@@ -110,7 +147,7 @@ class HODBatchStep(object):
         self.res = D.prepare(E.HP_MARKED_NPAIRS_3D, batch.mocks, batch.cols, batch.cols,
                              batch.rbins * batch.rbins, same=True, w1=self.w, nw=1, wfunc=1,
                              want_counts=False, want_sums=True, rows1=True, rows2=True, also_counts=True,
-                             device=dev)
+                             device=dev, tile_waves=batch.tile_waves)
         self.graph = None
 
     def _body(self):

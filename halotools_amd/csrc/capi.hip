@@ -147,6 +147,7 @@ struct Plan {
     size_t o_pcount = 0, o_psum = 0, o_pevals = 0, o_red = 0, o_status = 0;
     bool elim = false;
     size_t o_corr = 0, o_rows_tmp = 0;
+    bool sym = false;  // unweighted auto-count: symmetric evaluation
     size_t total = 0;
     std::vector<unsigned char> consts;  // bytes [0, o_gm[0])
 };
@@ -154,6 +155,7 @@ struct Plan {
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 constexpr double kCellTarget = 8.0;  // default stream points per fine cell
+constexpr int kHilbertMaxBits = 8;   // lane-ordering grid <= 256^3 cells per mock
 
 static int bits_for(uint64_t range) {
     int b = 1;
@@ -175,6 +177,8 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     P.sums = P.marked && !(r->flags & HP_NO_SUMS);
     P.counts = !P.marked || (r->flags & HP_ALSO_COUNTS);
     P.nw = P.marked ? r->nw : 0;
+    if (r->tile_waves != 0 && r->tile_waves != 1 && r->tile_waves != 2 && r->tile_waves != 4 && r->tile_waves != 8)
+        return fail("tile_waves must be 0, 1, 2, 4 or 8");
     if (P.marked) {
         if (r->wfunc < 1 || r->wfunc > 17) return fail("weight_func_id must be in 1..17");
         if (P.nw < 1 || P.nw > 5) return fail("weights per point must be in 1..5");
@@ -229,7 +233,6 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
         const int64_t nm[2] = {hm.n1, P.same ? hm.n1 : hm.n2};
         if (nm[0] < 0 || nm[1] < 0) return fail("negative point count");
         double Lmax = 0.0;
-        int maxdiv = 1;
         int64_t ncells = 1;
         // default fine-cell edge: about kCellTarget stream points per cell, kept
         // within [R/3, R]; finer cells tighten the pair windows, coarser cells
@@ -252,12 +255,14 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
             G.cell[d] = L / (double)G.ndiv[d];
             G.period[d] = L;
             Lmax = std::max(Lmax, L);
-            maxdiv = std::max(maxdiv, G.ndiv[d]);
             ncells *= G.ndiv[d];
         }
         G.pbc = hm.pbc ? 1 : 0;
-        int mb = 0;
-        while ((1 << mb) < maxdiv) mb++;
+        // lane-ordering grid: about one point per Hilbert cell, so a run of 64
+        // consecutive points follows the curve closely and stays compact
+        const double nlane = (double)std::max(nm[0], nm[1]);
+        int mb = 1;
+        while (mb < kHilbertMaxBits && (double)(1 << mb) < 1.25 * cbrt(nlane)) mb++;
         G.mbits = mb;
         const float eps = (float)(1e-5 * Lmax);
         G.eps = eps;
@@ -280,6 +285,10 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
         cell_off += ncells;
         key_off += (int64_t)1 << (3 * mb);
         if (nm[1] > 0 && (double)nm[1] * 27.0 >= 4.0e9) return fail("too many points in one mock for 32-bit histograms");
+        // the pair kernel addresses a mock's stream points (and weights) with 32-bit byte offsets
+        for (int s = 0; s < 2; s++)
+            if ((double)nm[s] * 8.0 * (double)std::max(P.nw, 1) >= 4294967296.0)
+                return fail("too many points (times weights per point) in one mock for 32-bit offsets");
     }
     P.n[0] = p_off[0];
     P.n[1] = p_off[1];
@@ -324,10 +333,12 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
             if (!S.needed) continue;
             const size_t n = (size_t)std::max<int64_t>(P.n[s], 1);
             nmax = std::max<int64_t>(nmax, (int64_t)n);
-            S.x = take(n * 8);
-            S.y = take(n * 8);
-            S.z = take(n * 8);
-            if (P.marked) S.w = take(n * 8 * (size_t)P.nw);
+            if (o == kRowMajor) {
+                S.x = take(n * 8);
+                S.y = take(n * 8);
+                S.z = take(n * 8);
+                if (P.marked) S.w = take(n * 8 * (size_t)P.nw);
+            }
             S.perm = take(n * 4);
             if (o == kRowMajor) S.cs = take((size_t)(P.total_cells + 1) * 4);
         }
@@ -352,6 +363,9 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
         P.o_corr = take((size_t)std::max<int64_t>(P.n[0], 1) * (size_t)P.Kr * 8);
         if (!P.rows1) P.o_rows_tmp = take((size_t)std::max<int64_t>(P.n[0], 1) * (size_t)P.Kr * 8);
     }
+    // unweighted auto-counts evaluate each unshifted unordered pair once; the
+    // Hilbert ranks reuse the sort scratch (free once both sides are built)
+    P.sym = P.same && !P.marked;
     P.total = align_up(off, 256);
 
     // ---- constant blob (geometry, edges, tables) that goes to the workspace
@@ -370,12 +384,23 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
 
 static SortedSide side_ptrs(char* ws, const Side& S) {
     SortedSide o;
-    o.x = reinterpret_cast<double*>(ws + S.x);
-    o.y = reinterpret_cast<double*>(ws + S.y);
-    o.z = reinterpret_cast<double*>(ws + S.z);
+    o.x = S.x ? reinterpret_cast<double*>(ws + S.x) : nullptr;
+    o.y = S.y ? reinterpret_cast<double*>(ws + S.y) : nullptr;
+    o.z = S.z ? reinterpret_cast<double*>(ws + S.z) : nullptr;
     o.w = S.w ? reinterpret_cast<double*>(ws + S.w) : nullptr;
     o.perm = reinterpret_cast<uint32_t*>(ws + S.perm);
     o.cell_start = S.cs ? reinterpret_cast<uint32_t*>(ws + S.cs) : nullptr;
+    return o;
+}
+
+// Lane side: the input arrays, read through the Hilbert permutation.
+static SortedSide lane_ptrs(char* ws, const Side& S, const double* x, const double* y, const double* z,
+                            const double* w) {
+    SortedSide o = side_ptrs(ws, S);
+    o.x = const_cast<double*>(x);
+    o.y = const_cast<double*>(y);
+    o.z = const_cast<double*>(z);
+    o.w = const_cast<double*>(w);
     return o;
 }
 
@@ -413,6 +438,7 @@ static int run(const hp_request* r, int kind) {
     sb.vals[1] = reinterpret_cast<uint32_t*>(ws + P.o_vals[1]);
     sb.tile_counts = reinterpret_cast<uint32_t*>(ws + P.o_tiles);
     sb.digit_totals = reinterpret_cast<uint32_t*>(ws + P.o_digits);
+    if (r->build_events[0]) hipEventRecord(static_cast<hipEvent_t>(r->build_events[0]), s);
     for (int sd = 0; sd < 2; sd++) {
         for (int o = 0; o < 2; o++) {
             const Side& S = P.sorted[sd][o];
@@ -424,6 +450,16 @@ static int run(const hp_request* r, int kind) {
                 return fail("cell build launch failed");
         }
     }
+    const uint32_t* hrank = nullptr;
+    if (P.sym && P.n[0] > 0) {
+        uint32_t* hr = reinterpret_cast<uint32_t*>(ws + P.o_keys[1]);
+        if (launch_hilbert_rank(reinterpret_cast<const uint32_t*>(ws + P.sorted[0][kHilbert].perm),
+                                reinterpret_cast<const uint32_t*>(ws + P.sorted[0][kRowMajor].perm), P.n[0],
+                                reinterpret_cast<uint32_t*>(ws + P.o_keys[0]), hr, s))
+            return fail("rank launch failed");
+        hrank = hr;
+    }
+    if (r->build_events[1]) hipEventRecord(static_cast<hipEvent_t>(r->build_events[1]), s);
 
     const int s2 = P.same ? 0 : 1;
     int32_t* gm0 = reinterpret_cast<int32_t*>(ws + P.o_gm[0]);
@@ -434,13 +470,15 @@ static int run(const hp_request* r, int kind) {
     L.geom = dgeom;
     L.group_mock = gm0;
     L.ngroups = P.ngroups[0];
-    L.lane = side_ptrs(ws, P.sorted[0][kHilbert]);
+    L.lane = lane_ptrs(ws, P.sorted[0][kHilbert], X[0], Y[0], Z[0], P.marked ? W[0] : nullptr);
     L.strm = side_ptrs(ws, P.sorted[s2][kRowMajor]);
     L.nw = P.nw;
     L.br = P.tr.spec;
     L.bp = P.tp.spec;
     L.ws = ws;
     L.wfunc = r->wfunc;
+    L.waves = r->tile_waves;
+    L.hrank = hrank;
     L.pcount = P.counts ? reinterpret_cast<uint64_t*>(ws + P.o_pcount) : nullptr;
     L.psum = P.sums ? reinterpret_cast<double*>(ws + P.o_psum) : nullptr;
     L.pevals = reinterpret_cast<uint64_t*>(ws + P.o_pevals);
@@ -481,12 +519,13 @@ static int run(const hp_request* r, int kind) {
         B.also_count = 0;
         B.group_mock = gm1;
         B.ngroups = P.ngroups[s2];
-        B.lane = side_ptrs(ws, P.sorted[s2][kHilbert]);
+        B.lane = lane_ptrs(ws, P.sorted[s2][kHilbert], X[s2], Y[s2], Z[s2], W[s2]);
         B.strm = side_ptrs(ws, P.sorted[0][kRowMajor]);
         B.rows = r->rows2;
         B.pcount = nullptr;
         B.psum = nullptr;
         B.pevals = nullptr;
+        B.hrank = nullptr;
         // with SAME_SAMPLES the side-1 rows of the geometry equal the side-0 rows
         if (launch_pairs(B, s)) return fail("pair kernel (swapped) launch failed");
     }

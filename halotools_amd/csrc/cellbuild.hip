@@ -248,6 +248,29 @@ __global__ void k_cell_start(const uint32_t* __restrict__ keys, int64_t n, int64
 
 static inline int check(const char*) { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
+// Position in the Hilbert (lane) order of every point of the row-major
+// (stream) order of the same sample: hpos = inverse of the Hilbert
+// permutation, hrank[j] = hpos[row_perm[j]].
+__global__ void k_inverse_perm(const uint32_t* __restrict__ perm, int64_t n, uint32_t* __restrict__ inv) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) inv[perm[p]] = (uint32_t)p;
+}
+
+__global__ void k_compose_perm(const uint32_t* __restrict__ inv, const uint32_t* __restrict__ perm, int64_t n,
+                               uint32_t* __restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) out[j] = inv[perm[j]];
+}
+
+int launch_hilbert_rank(const uint32_t* hil_perm, const uint32_t* row_perm, int64_t n, uint32_t* hpos,
+                        uint32_t* hrank, hipStream_t s) {
+    if (n <= 0) return 0;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_inverse_perm, dim3(nb), dim3(256), 0, s, hil_perm, n, hpos);
+    hipLaunchKernelGGL(k_compose_perm, dim3(nb), dim3(256), 0, s, hpos, row_perm, n, hrank);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, const double* x, const double* y,
                       const double* z, const double* w, int nw, int64_t n, int64_t total_cells, int key_bits,
                       SortBufs sb, SortedSide out, int32_t* status, hipStream_t s) {
@@ -275,8 +298,11 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, c
         cur ^= 1;
     }
     if (check("sort")) return -1;
-    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, s, out.perm, n, x, y, z, w, nw, out.x, out.y, out.z,
-                       out.w);
+    // The lane side (Hilbert order) is read once per group through the
+    // permutation; only the stream side needs sorted copies.
+    if (order == kRowMajor)
+        hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, s, out.perm, n, x, y, z, w, nw, out.x, out.y, out.z,
+                           out.w);
     if (order == kRowMajor) {
         hipLaunchKernelGGL(k_cell_start, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, sb.keys[cur], n,
                            total_cells, out.cell_start);

@@ -86,6 +86,8 @@ int launch_store_bytes(void* dst, const void* src, size_t nbytes, hipStream_t s)
 int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 int launch_group_mock(const MockGeom* g, int nmocks, int side, int32_t* group_mock,
                       int64_t ngroups_total, hipStream_t s);
+int launch_hilbert_rank(const uint32_t* hil_perm, const uint32_t* row_perm, int64_t n, uint32_t* hpos,
+                        uint32_t* hrank, hipStream_t s);
 int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order,
                       const double* x, const double* y, const double* z, const double* w,
                       int nw, int64_t n, int64_t total_cells, int key_bits,
@@ -97,10 +99,11 @@ struct PairLaunch {
     int swap;              // 1: lanes are sample 2
     int also_count;        // marked: also accumulate unweighted counts
     int wfunc;             // marked weight function id
+    int waves;             // wavefronts per workgroup (1, 2, 4, 8); 0 = default
     const MockGeom* geom;
     const int32_t* group_mock;
     int64_t ngroups;
-    SortedSide lane;       // Hilbert-sorted lane side
+    SortedSide lane;       // lane side: input-order x/y/z/w plus the Hilbert permutation
     SortedSide strm;       // row-major sorted stream side
     int nw;
     BinSpec br;            // radial / rp bins
@@ -111,6 +114,9 @@ struct PairLaunch {
     double* rows;          // [n_lane_total][nr] (original order) or nullptr
     double* corr;          // auto-counts: rows2 - rows1 corrections from seam pairs, or nullptr
     uint64_t* pevals;      // [ngroups]
+    // auto-counts (unweighted): Hilbert position of each stream point; each
+    // unshifted unordered pair is then evaluated once and counted twice
+    const uint32_t* hrank;
 };
 enum Mode : int { kModeCount3D = 0, kModeProj = 1, kModeXYZ = 2, kModeWSum = 3, kModeJac = 4 };
 int launch_pairs(const PairLaunch& L, hipStream_t s);

@@ -140,16 +140,40 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb) {
 #endif
 constexpr int kBatch = HP_BATCH;  // stream points per scalar-load wait
 
+// Read-only global data seen through the constant address space: uniform
+// loads from it always become scalar loads (SGPR results, no VGPRs).
+typedef const __attribute__((address_space(4))) char cchar;
+__device__ __forceinline__ double sload_f64(const void* base, uint32_t byte_off) {
+    cchar* const p = (cchar*)(const char*)base + byte_off;  // address-space cast
+    return *(const __attribute__((address_space(4))) double*)p;
+}
+
+__device__ __forceinline__ double sload_f64_at(const double* base, int64_t i) {
+    return ((const __attribute__((address_space(4))) double*)(const char*)base)[i];
+}
+__device__ __forceinline__ uint32_t sload_u32_at(const uint32_t* base, int64_t i) {
+    return ((const __attribute__((address_space(4))) uint32_t*)(const char*)base)[i];
+}
+
 struct PairArgs {
     PairLaunch L;
     int wave_bytes;     // histogram bytes per wave
     int cnt_off;        // marked + counts: byte offset of the count rows
     int shared_count;   // 1: one histogram per wave instead of per lane
     int smem;           // dynamic LDS bytes
+    int ring_off;       // byte offset of the survivor ring within a wave's LDS
 };
 
+// Survivor ring per wave: up to B-1 leftover entries plus one 64-point chunk.
+constexpr int kRing = 72;
+
+// Occupancy the register allocation is held to (waves per SIMD): the count
+// kernels are limited by VGPRs otherwise, the FP64-histogram kernels by LDS.
+template <int MODE>
+constexpr int kWavesPerSimd = (MODE == kModeWSum || MODE == kModeJac) ? 1 : (MODE == kModeXYZ ? 5 : 6);
+
 template <int MODE, int SWAP, int ALSO, int MULTI, int WAVES>
-__global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
+__global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const PairArgs A) {
     constexpr bool F64 = (MODE == kModeWSum || MODE == kModeJac);
     __shared__ LdsEntry s_tr[kTableMax];
     __shared__ LdsEntry s_tp[MODE == kModeXYZ ? kTableMax : 1];
@@ -205,11 +229,13 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
     const int cnt = uni((int)(left < kGroup ? left : kGroup));
     const bool act = lane < cnt;
     const int64_t p = p0 + (act ? lane : 0);
+    // lane points are read in input order through the Hilbert permutation
+    const int64_t q = (int64_t)L.lane.perm[p];
     // inactive lanes carry NaN: every comparison fails, so they land in the trash row
     const double qnan = __builtin_nan("");
-    const double px = act ? L.lane.x[p] : qnan;
-    const double py = act ? L.lane.y[p] : qnan;
-    const double pz = act ? L.lane.z[p] : qnan;
+    const double px = act ? L.lane.x[q] : qnan;
+    const double py = act ? L.lane.y[q] : qnan;
+    const double pz = act ? L.lane.z[q] : qnan;
 
     const double Lx = G.period[0], Ly = G.period[1], Lz = G.period[2];
     const int nx = G.ndiv[0], ny = G.ndiv[1], nz = G.ndiv[2];
@@ -229,6 +255,7 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
     const float Rx = G.search[0], Ry = G.search[1], Rz = G.search[2];
     const float cxf = (float)G.cell[0], cyf = (float)G.cell[1], czf = (float)G.cell[2];
     const float r2xy = G.r2xy;
+    const int shape = uni(G.shape);
 
     int jx0 = (int)floorf((bxlo - Rx) / cxf), jx1 = (int)floorf((bxhi + Rx) / cxf);
     int jy0 = (int)floorf((bylo - Ry) / cyf), jy1 = (int)floorf((byhi + Ry) / cyf);
@@ -245,7 +272,7 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
     double lw[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     const int nw = L.nw;
     if (F64) {
-        for (int c = 0; c < nw && c < 5; c++) lw[c] = L.lane.w[p * nw + c];
+        for (int c = 0; c < nw && c < 5; c++) lw[c] = L.lane.w[q * nw + c];
     }
     const double* __restrict__ SX = L.strm.x;
     const double* __restrict__ SY = L.strm.y;
@@ -305,17 +332,35 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
             return bin_addr<MULTI>(dsq, s_tr, shr, bshr, nbr1, step_r, gedges_r);
         }
     };
-    auto accumulate = [&](uint32_t a, int j) __attribute__((always_inline)) {
+    // stream points of this group's mock, addressed by 32-bit offsets from
+    // the mock's first point (the ring holds mock-relative indices)
+    const int64_t sbase = G.p_off[SWAP ? 0 : 1];
+    const double* const SXm = SX + sbase;
+    const double* const SYm = SY + sbase;
+    const double* const SZm = SZ + sbase;
+    const double* const SWm = F64 ? SW + sbase * nw : nullptr;
+    // Unweighted auto-count: an unshifted pair has dx_ji = -dx_ij exactly
+    // (IEEE subtraction is antisymmetric), so both ordered pairs bin alike.
+    // Unshifted stream points are then taken only when they follow the whole
+    // group in the lane (Hilbert) order and count twice; the group's own
+    // points are paired separately below; shifted images (whose two orders
+    // may round apart) are evaluated from both sides, once each.
+    const uint32_t* __restrict__ HR = L.hrank;
+    const bool sym = !F64 && !SWAP && HR != nullptr;
+    const uint32_t hend = (uint32_t)(p0 + cnt);
+    // r: mock-relative stream index
+    // inc: count-mode increment (2 for an unshifted pair evaluated once for both orders)
+    auto accumulate = [&](uint32_t a, uint32_t r, uint32_t inc) __attribute__((always_inline)) {
         if (!F64) {
-            __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(hl + a), 1u, __ATOMIC_RELAXED,
+            __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(hl + a), inc, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
             double v;
             if (MODE == kModeJac) {
-                v = SW[j];
+                v = sload_f64(SWm, r * 8u);
             } else {
                 double sw[5];
-                for (int c = 0; c < nw && c < 5; c++) sw[c] = SW[(int64_t)j * nw + c];
+                for (int c = 0; c < nw && c < 5; c++) sw[c] = sload_f64(SWm, (r * (uint32_t)nw + (uint32_t)c) * 8u);
                 v = wfunc_eval(wfunc, lw, sw);
             }
             __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a), v, __ATOMIC_RELAXED,
@@ -327,6 +372,57 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
         }
     };
 
+    // ---- survivor ring: stream indices that passed the point filter, all
+    // with the same periodic image (the ring is drained when the image changes)
+    constexpr int B = kBatch;
+    uint32_t* const ring = reinterpret_cast<uint32_t*>(wsm + A.ring_off);
+    int qlen = 0;
+    int rimg = -1;
+    uint32_t rinc = 1u;  // count increment of the ring's entries
+    double rshx = 0.0, rshy = 0.0, rshz = 0.0;
+    int rsxs = 0, rsys = 0, rszs = 0;
+    double rax = px, ray = py, raz = pz;
+    // evaluate ring[q0, q0 + nv), nv <= B, against the 64 lane points: the
+    // indices arrive with one LDS read, the points with scalar loads at
+    // 32-bit offsets (one shift serves x, y and z)
+    auto consume = [&](int q0, int nv) __attribute__((always_inline)) {
+        const int v = (int)ring[q0 + min(lane, nv - 1)];
+        uint32_t off[B];
+#pragma unroll
+        for (int u = 0; u < B; u++) off[u] = (uint32_t)__builtin_amdgcn_readlane(v, u) * 8u;
+        double bx[B], by[B], bz[B];
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+            bx[u] = sload_f64(SXm, off[u]);
+            by[u] = sload_f64(SYm, off[u]);
+            bz[u] = sload_f64(SZm, off[u]);
+        }
+        evals += (uint64_t)cnt * (uint64_t)nv;
+        uint32_t a[B];
+        if (nv == B) {
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                double dx, dy, dz;
+                sep(bx[u], by[u], bz[u], rshx, rshy, rshz, rsxs, rsys, rszs, rax, ray, raz, dx, dy, dz);
+                a[u] = addr_of(dx, dy, dz);
+            }
+        } else {
+            // short batch: the repeated slots go to the trash row
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                double dx, dy, dz;
+                sep(bx[u], by[u], bz[u], rshx, rshy, rshz, rsxs, rsys, rszs, rax, ray, raz, dx, dy, dz);
+                a[u] = (u < nv) ? addr_of(dx, dy, dz) : trash_r;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < B; u++) accumulate(a[u], off[u] >> 3, rinc);
+    };
+    auto drain = [&]() __attribute__((always_inline)) {
+        for (int q = 0; q < qlen; q += B) consume(q, min(B, qlen - q));
+        qlen = 0;
+    };
+
     for (int jx = jx0; jx <= jx1; jx++) {
         const float gx0 = fmaxf(0.f, fmaxf((float)jx * cxf - bxhi, bxlo - (float)(jx + 1) * cxf) - eps);
         const int tx = jx < 0 ? -1 : (jx >= nx ? 1 : 0);
@@ -335,7 +431,7 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
             const float gy0 = fmaxf(0.f, fmaxf((float)jy * cyf - byhi, bylo - (float)(jy + 1) * cyf) - eps);
             const float d2 = gx0 * gx0 + gy0 * gy0;
             if (d2 > r2xy) continue;
-            const float dzr = (G.shape == 0) ? sqrtf(fmaxf(r2xy - d2, 0.f)) + eps : Rz;
+            const float dzr = (shape == 0) ? sqrtf(fmaxf(r2xy - d2, 0.f)) + eps : Rz;
             int jz0 = (int)floorf((bzlo - dzr) / czf), jz1 = (int)floorf((bzhi + dzr) / czf);
             if (pbc) { jz0 = max(jz0, -nz); jz1 = min(jz1, 2 * nz - 1); }
             else { jz0 = max(jz0, 0); jz1 = min(jz1, nz - 1); }
@@ -356,9 +452,8 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
                 const int jb = uni((int)CS[colbase + w0]);
                 const int je = uni((int)CS[colbase + w1 + 1]);
                 if (jb >= je) continue;
-                evals += (uint64_t)cnt * (uint64_t)(je - jb);
                 if (shared) {
-                    bound += (uint64_t)cnt * (uint64_t)(je - jb);
+                    bound += (uint64_t)cnt * (uint64_t)(je - jb) * (sym ? 2u : 1u);
                     if (bound >= 0xF0000000ull) {
                         for (int s = lane; s < shared_bins; s += 64) {
                             const int src = (MODE == kModeXYZ) ? (s / Kp) * row_len + s % Kp : s;
@@ -383,7 +478,6 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
                     if (sys) ay = py - shy;
                     if (szs) az = pz - shz;
                 }
-                constexpr int B = kBatch;
                 int j = jb;
                 if (MODE == kModeJac && !SWAP && L.corr && (sxs || sys || szs)) {
                     // Auto-count with one set of weights: the swapped orientation
@@ -392,6 +486,7 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
                     // orientations may round apart; record the rare disagreement
                     // as a correction so rows2 = rows1 + corr exactly.
                     const uint32_t* __restrict__ SP = L.strm.perm;
+                    evals += (uint64_t)cnt * (uint64_t)(je - jb);
                     for (; j < je; j++) {
                         const double qx = SX[j], qy = SY[j], qz = SZ[j];
                         double dx, dy, dz;
@@ -401,7 +496,7 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
                         const double ry = sys ? (qy + shy) - py : dy;
                         const double rz = szs ? (qz + shz) - pz : dz;
                         const uint32_t ar = addr_of(rx, ry, rz);
-                        accumulate(a, j);
+                        accumulate(a, (uint32_t)(j - sbase), 1u);
                         if (ar != a) {
                             const uint32_t sf = a / step_r, sr = ar / step_r;
                             double* crow = L.corr + (int64_t)SP[j] * Kr;
@@ -412,30 +507,91 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
                         }
                     }
                 }
-                for (; j + B <= je; j += B) {
-                    double bx[B], by[B], bz[B];
-#pragma unroll
-                    for (int u = 0; u < B; u++) {
-                        bx[u] = SX[j + u];
-                        by[u] = SY[j + u];
-                        bz[u] = SZ[j + u];
-                    }
-                    uint32_t a[B];
-#pragma unroll
-                    for (int u = 0; u < B; u++) {
-                        double dx, dy, dz;
-                        sep(bx[u], by[u], bz[u], shx, shy, shz, sxs, sys, szs, ax, ay, az, dx, dy, dz);
-                        a[u] = addr_of(dx, dy, dz);
-                    }
-#pragma unroll
-                    for (int u = 0; u < B; u++) accumulate(a[u], j + u);
+                if (j >= je) continue;
+
+                // Point filter: 64 stream points at a time (one per lane, one
+                // vector load) are tested against the group's box grown by the
+                // search length; survivors go to the ring and are evaluated
+                // against all 64 lane points in batches of B.  The image of
+                // stream point q visited here sits at q + t*L for either
+                // orientation.  f32 with the box rounded outwards and an eps
+                // margin keeps every in-range pair.
+                const int img = (sxs + 1) * 9 + (sys + 1) * 3 + (szs + 1);
+                if (img != rimg) {
+                    drain();
+                    rimg = img;
+                    rshx = shx; rshy = shy; rshz = shz;
+                    rsxs = sxs; rsys = sys; rszs = szs;
+                    rax = ax; ray = ay; raz = az;
+                    rinc = (sym && img == 13) ? 2u : 1u;
                 }
-                for (; j < je; j++) {
-                    double dx, dy, dz;
-                    sep(SX[j], SY[j], SZ[j], shx, shy, shz, sxs, sys, szs, ax, ay, az, dx, dy, dz);
-                    accumulate(addr_of(dx, dy, dz), j);
+                const bool after_group = sym && img == 13;
+                const float ftx = (float)(tx * Lx), fty = (float)(ty * Ly), ftz = (float)(tz * Lz);
+                for (int c = j; c < je; c += 64) {
+                    const int jj = c + lane;
+                    bool ok = false;
+                    if (jj < je) {
+                        const float fx = (float)SX[jj] + ftx;
+                        const float fy = (float)SY[jj] + fty;
+                        const float fz = (float)SZ[jj] + ftz;
+                        const float gx = fmaxf(0.f, fmaxf(fx - bxhi, bxlo - fx) - eps);
+                        const float gy = fmaxf(0.f, fmaxf(fy - byhi, bylo - fy) - eps);
+                        const float gz = fmaxf(0.f, fmaxf(fz - bzhi, bzlo - fz) - eps);
+                        const float gxy = gx * gx + gy * gy;
+                        ok = (shape == 0) ? (gxy + gz * gz <= r2xy) : (gxy <= r2xy && gz <= Rz);
+                        if (after_group) ok = ok && HR[jj] >= hend;
+                    }
+                    const uint64_t m = (uint64_t)__ballot(ok);
+                    if (ok) {
+                        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        ring[qlen + (int)rank] = (uint32_t)(jj - sbase);
+                    }
+                    // lanes read each other's ring slots: keep LDS accesses in order
+                    __builtin_amdgcn_wave_barrier();
+                    qlen += __builtin_popcountll(m);
+                    int qh = 0;
+                    for (; qlen - qh >= B; qh += B) consume(qh, B);
+                    if (qh > 0) {
+                        // move the < B leftovers to the front of the ring
+                        const int r = qlen - qh;
+                        if (r > 0) {
+                            const uint32_t t = ring[qh + min(lane, r - 1)];
+                            if (lane < r) ring[lane] = t;
+                            __builtin_amdgcn_wave_barrier();
+                        }
+                        qlen = r;
+                    }
                 }
             }
+        }
+    }
+    drain();
+
+    if (sym) {
+        // the group's own points as stream points, unshifted: lane i with
+        // point k counts 2 when k follows i in the lane order, 1 for k == i
+        // (the self pair), 0 when k precedes i (counted from the other side)
+        for (int k0 = 0; k0 < cnt; k0 += B) {
+            double bx[B], by[B], bz[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                const int64_t qk = (int64_t)sload_u32_at(L.lane.perm, p0 + min(k0 + u, cnt - 1));
+                bx[u] = sload_f64_at(L.lane.x, qk);
+                by[u] = sload_f64_at(L.lane.y, qk);
+                bz[u] = sload_f64_at(L.lane.z, qk);
+            }
+            evals += (uint64_t)cnt * (uint64_t)min(B, cnt - k0);
+            uint32_t a[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                double dx, dy, dz;
+                sep(bx[u], by[u], bz[u], 0.0, 0.0, 0.0, 0, 0, 0, px, py, pz, dx, dy, dz);
+                const int k = k0 + u;
+                a[u] = (k < cnt && k >= lane) ? addr_of(dx, dy, dz) : trash_r;
+            }
+#pragma unroll
+            for (int u = 0; u < B; u++) accumulate(a[u], 0u, (k0 + u > lane) ? 2u : 1u);
         }
     }
 
@@ -463,8 +619,7 @@ __global__ void __launch_bounds__(WAVES * 64) k_pairs(const PairArgs A) {
     }
     if (MODE == kModeJac) {
         if (L.rows && act) {
-            const int64_t orig = (int64_t)L.lane.perm[p];
-            double* row = L.rows + orig * Kr;
+            double* row = L.rows + q * Kr;
             for (int s = 0; s < Kr; s++) row[s] = fh[s * 64 + lane];
         }
         if (L.psum && !SWAP) {
@@ -524,12 +679,13 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
         default: return -1;
     }
     if (L.br.nb > kTableMax || L.bp.nb > kTableMax) return -1;
-    A.wave_bytes = align16(wave_bytes);
+    A.ring_off = align16(wave_bytes);
+    A.wave_bytes = align16(A.ring_off + kRing * 4);
     const int static_lds = (int)sizeof(LdsEntry) * kTableMax * (L.mode == kModeXYZ ? 2 : 1);
 #ifndef HP_MAX_WAVES
 #define HP_MAX_WAVES 8
 #endif
-    int waves = HP_MAX_WAVES;
+    int waves = L.waves > 0 ? L.waves : HP_MAX_WAVES;
     while (waves > 1 && static_lds + waves * A.wave_bytes > 72 * 1024) waves >>= 1;
     A.smem = waves * A.wave_bytes;
     if (static_lds + A.smem > 160 * 1024) return -2;

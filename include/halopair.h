@@ -84,7 +84,9 @@ typedef struct hp_request {
     int32_t nw;                /* weights per point (id 1,2: 1; 3-11: 2; 12,13: 4; ...) */
     int32_t nr;                /* number of radial / rp edges                          */
     int32_t np;                /* xy_z: number of pi edges; projected: 1               */
-    int32_t reserved;
+    int32_t tile_waves;        /* pair-kernel workgroup: 1, 2, 4 or 8 wavefronts of 64 */
+                               /* lane points; 0 = library default (8, fewer if the    */
+                               /* histograms do not fit in LDS)                        */
     const double* r2;          /* host [nr]: rbins*rbins (or rp_bins*rp_bins)          */
     const double* p2;          /* host [np]: pi_bins*pi_bins, or {pi_max*pi_max}        */
     double search[3];          /* mesh search lengths (rmax,rmax,rmax) / (rp,rp,pi)     */
@@ -100,6 +102,8 @@ typedef struct hp_request {
     void* stream;              /* hipStream_t; NULL = default stream                   */
     void* pair_events[2];      /* optional hipEvent_t pair recorded on `stream` right  */
                                /* before and after the pair kernel(s) (profiling)      */
+    void* build_events[2];     /* optional hipEvent_t pair recorded right before and   */
+                               /* after the cell build (profiling)                     */
 } hp_request;
 
 /* Thread-local message describing the last error (empty string if none). */

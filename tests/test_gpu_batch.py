@@ -80,3 +80,34 @@ def test_hod_step_eager_and_graph(data, gpu):
         assert torch.equal(g_counts, e_counts)
         assert torch.allclose(g_W, e_W, rtol=1e-14, atol=0)
         assert torch.allclose(g_grad, e_grad, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
+def test_tile_waves_do_not_change_results(data, gpu, waves):
+    # the workgroup shape only regroups lane groups: counts and per-group
+    # partial sums (reduced in a fixed order) are bit-identical
+    from halotools_amd.batch import MockBatch
+    sizes, pts, lm, logMmin, c = data
+    ref = MockBatch(pts, L, RB, device=gpu, cells=[5.0] * len(sizes))
+    b = MockBatch(pts, L, RB, device=gpu, cells=[5.0] * len(sizes), tile_waves=waves)
+    assert torch.equal(b.npairs_3d(), ref.npairs_3d())
+    w = torch.cat([0.5 + 0.1 * torch.sin(x) for x in lm])
+    assert torch.equal(b.marked_npairs_3d_product(w), ref.marked_npairs_3d_product(w))
+
+
+def test_autotune_keeps_results(data, gpu, tmp_path, monkeypatch):
+    from halotools_amd import autotune
+    from halotools_amd import _engine as E
+    from halotools_amd.batch import HODBatchStep, MockBatch
+    monkeypatch.setenv("HALOTOOLS_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(autotune, "_CACHE", {})
+    sizes, pts, lm, logMmin, c = data
+    b = MockBatch(pts, L, RB, device=gpu)
+    before = b.npairs_3d().clone()
+    tuned = b.tune(E.HP_MARKED_NPAIRS_3D, reps=1, targets=(2.0, 8.0), waves=(2, 8))
+    assert tuned and (tmp_path / "tune.json").exists()
+    assert b.tile_waves in (2, 8)
+    assert torch.equal(b.npairs_3d(), before)
+    step = HODBatchStep(b, torch.cat(lm), logMmin, 0.3, c)
+    counts = step.run()[0]
+    assert torch.equal(counts, before)

@@ -44,7 +44,31 @@ def test_abi_struct_sizes():
     # hp_mock and hp_request must match the C layout of include/halopair.h
     assert ctypes.sizeof(E.HpMock) == 8 + 8 + 24 + 4 + 4 + 24
     off = E.HpRequest.workspace.offset
-    assert off % 8 == 0 and ctypes.sizeof(E.HpRequest) == E.HpRequest.pair_events.offset + 16
+    assert off % 8 == 0 and ctypes.sizeof(E.HpRequest) == E.HpRequest.build_events.offset + 16
+
+
+def test_abi_layout_matches_c_compiler(tmp_path):
+    # the ctypes mirror against what a C compiler makes of include/halopair.h
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "halopair.h"', "int main(void) {"]
+    for cname, cls in (("hp_mock", E.HpMock), ("hp_request", E.HpRequest)):
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for field in cls._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, field[0], cname, field[0]))
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                         text=True).stdout.splitlines())
+    for cname, cls in (("hp_mock", E.HpMock), ("hp_request", E.HpRequest)):
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for field in cls._fields_:
+            assert int(got["%s.%s" % (cname, field[0])]) == getattr(cls, field[0]).offset, (cname, field[0])
 
 
 @pytest.mark.parametrize("edges", [
