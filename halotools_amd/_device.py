@@ -192,3 +192,25 @@ def backward_rows(rows, gsuffix, offsets, out=None):
         len(offsets) - 1, ctypes.c_void_p(grad.data_ptr()),
         ctypes.c_void_p(torch.cuda.current_stream(rows.device).cuda_stream)))
     return grad
+
+
+class SegmentSum(object):
+    """Per-mock sums out[m] = values[offsets[m]:offsets[m+1]].sum() in a fixed
+    order (library kernels, no atomics), with the scratch allocated once so the
+    call can be captured in a HIP graph."""
+
+    def __init__(self, offsets, device):
+        self.offsets = (ctypes.c_int64 * len(offsets))(*[int(v) for v in offsets])
+        self.nmocks = len(offsets) - 1
+        lib = E.library()
+        need = lib.hp_segment_sum_scratch(self.offsets, self.nmocks)
+        if need < 0:
+            E.check(-1)
+        self.scratch = torch.empty(int(need), dtype=torch.float64, device=device)
+
+    def __call__(self, values, out):
+        E.check(E.library().hp_segment_sum(
+            ctypes.c_void_p(values.data_ptr()), self.offsets, self.nmocks, ctypes.c_void_p(out.data_ptr()),
+            ctypes.c_void_p(self.scratch.data_ptr()), self.scratch.numel(),
+            ctypes.c_void_p(torch.cuda.current_stream(values.device).cuda_stream)))
+        return out

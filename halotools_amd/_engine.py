@@ -27,7 +27,7 @@ HP_ALSO_COUNTS = 16
 EXPORTED_SYMBOLS = (
     "hp_last_error", "hp_abi_version", "hp_workspace_size", "hp_npairs_3d",
     "hp_npairs_projected", "hp_npairs_xy_z", "hp_marked_npairs_3d", "hp_marked_backward",
-    "hp_debug_bin_lookup", "hp_debug_hilbert",
+    "hp_segment_sum_scratch", "hp_segment_sum", "hp_debug_bin_lookup", "hp_debug_hilbert",
 )
 
 
@@ -102,6 +102,11 @@ def library():
                                        ctypes.POINTER(ctypes.c_int64), ctypes.c_int32,
                                        ctypes.c_void_p, ctypes.c_void_p]
     lib.hp_marked_backward.restype = ctypes.c_int
+    lib.hp_segment_sum_scratch.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]
+    lib.hp_segment_sum_scratch.restype = ctypes.c_int64
+    lib.hp_segment_sum.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+    lib.hp_segment_sum.restype = ctypes.c_int
     lib.hp_debug_bin_lookup.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int32,
                                         ctypes.POINTER(ctypes.c_double), ctypes.c_int64,
                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]

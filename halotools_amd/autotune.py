@@ -26,7 +26,7 @@ import torch
 from . import _device as D
 from . import _engine as E
 
-TARGETS = (2.0, 4.0, 8.0, 16.0, 32.0)
+TARGETS = (8.0, 32.0, 128.0, 4096.0)
 WAVES = (2, 4, 8)
 _CACHE = {}
 

@@ -12,8 +12,9 @@ come out of one fixed-order reduction.
 ``HODBatchStep`` is the model-fitting step of the benchmark: for every mock,
 the npairs_3d auto-counts, the product-weighted marked sums with Zheng07
 central-occupation weights, and the gradient of a loss sum_k c_k W_k with
-respect to each mock's logMmin -- counts and marked Jacobian rows in one pair
-traversal, then the backward contraction -- all capturable in one HIP graph.
+respect to each mock's logMmin -- a count pass, a marked pass producing the
+Jacobian rows, the backward contraction and a fixed-order per-mock sum -- all
+capturable in one HIP graph.
 """
 import math
 
@@ -144,22 +145,29 @@ class HODBatchStep(object):
         self.grad_w2 = torch.empty(n, dtype=torch.float64, device=dev)
         self.grad_logMmin = torch.zeros(batch.M, dtype=torch.float64, device=dev)
         self.G = suffix_sums(self.c).expand(batch.M, -1).contiguous()
+        # counts from the (symmetric) count pass; the marked pass keeps only
+        # its FP64 histograms in LDS, which leaves room for more waves per CU
+        self.res_counts = D.prepare(E.HP_NPAIRS_3D, batch.mocks, batch.cols, batch.cols,
+                                    batch.rbins * batch.rbins, same=True, device=dev)
         self.res = D.prepare(E.HP_MARKED_NPAIRS_3D, batch.mocks, batch.cols, batch.cols,
                              batch.rbins * batch.rbins, same=True, w1=self.w, nw=1, wfunc=1,
-                             want_counts=False, want_sums=True, rows1=True, rows2=True, also_counts=True,
+                             want_counts=False, want_sums=True, rows1=True, rows2=True,
                              device=dev, tile_waves=batch.tile_waves)
+        self.dtheta = torch.empty(n, dtype=torch.float64, device=dev)
+        self.segsum = D.SegmentSum(batch.offsets, dev)
         self.graph = None
 
     def _body(self):
         b = self.b
         u = (self.log_mass - self.logMmin[b.mock_id]) / self.sigma
         self.w[:, 0].copy_(0.5 * (1.0 + torch.erf(u)))
+        D.launch(self.res_counts)
         D.launch(self.res)
         D.backward_rows(self.res.rows1, self.G, b.offsets, out=self.grad_w1)
         D.backward_rows(self.res.rows2, self.G, b.offsets, out=self.grad_w2)
         dw = -torch.exp(-u * u) / (math.sqrt(math.pi) * self.sigma)
-        self.grad_logMmin.zero_()
-        self.grad_logMmin.index_add_(0, b.mock_id, (self.grad_w1 + self.grad_w2) * dw)
+        torch.mul(self.grad_w1 + self.grad_w2, dw, out=self.dtheta)
+        self.segsum(self.dtheta, self.grad_logMmin)
         self.loss = (self.res.sums * self.c).sum(dim=1)
 
     def run(self):
@@ -185,10 +193,10 @@ class HODBatchStep(object):
         return self.outputs()
 
     def outputs(self):
-        return self.res.counts, self.res.sums, self.loss, self.grad_logMmin
+        return self.res_counts.counts, self.res.sums, self.loss, self.grad_logMmin
 
     def status(self):
-        return int(self.res.cbuf[-1].item()) & 0xFFFFFFFF
+        return (int(self.res.cbuf[-1].item()) | int(self.res_counts.cbuf[-1].item())) & 0xFFFFFFFF
 
 
 def one_at_a_time(points, log_mass, logMmin, sigma, rbins, c, period):

@@ -154,7 +154,7 @@ struct Plan {
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-constexpr double kCellTarget = 8.0;  // default stream points per fine cell
+constexpr double kCellTarget = 4096.0;  // most stream points per default fine cell
 constexpr int kHilbertMaxBits = 8;   // lane-ordering grid <= 256^3 cells per mock
 
 static int bits_for(uint64_t range) {
@@ -234,9 +234,11 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
         if (nm[0] < 0 || nm[1] < 0) return fail("negative point count");
         double Lmax = 0.0;
         int64_t ncells = 1;
-        // default fine-cell edge: about kCellTarget stream points per cell, kept
-        // within [R/3, R]; finer cells tighten the pair windows, coarser cells
-        // cut the per-segment overhead of sparse samples
+        // default fine-cell edge: the search length.  The pair kernel culls
+        // stream points one by one against each lane group's box, so coarse
+        // cells cost only a cheap filter pass, while finer cells add segments
+        // (scalar round trips) per group; kCellTarget stream points per cell
+        // is the floor for very dense samples
         const double volume = hm.period[0] * hm.period[1] * hm.period[2];
         const double density = volume > 0.0 ? (double)nm[1] / volume : 0.0;
         const double natural = density > 0.0 ? cbrt(kCellTarget / density) : INFINITY;
@@ -574,6 +576,27 @@ int hp_marked_backward(const double* rows, int32_t nr, const double* gsuffix, co
     if (nr < 1 || nmocks < 1) return fail("bad sizes");
     if (launch_rows_dot(rows, nr, gsuffix, offsets, nmocks, grad, static_cast<hipStream_t>(stream)))
         return fail("backward launch failed");
+    g_err.clear();
+    return 0;
+}
+
+int64_t hp_segment_sum_scratch(const int64_t* offsets, int32_t nmocks) {
+    if (!offsets || nmocks < 1) return fail("bad arguments");
+    for (int m = 0; m < nmocks; m++)
+        if (offsets[m + 1] < offsets[m]) return fail("offsets must be non-decreasing");
+    g_err.clear();
+    return segment_sum_scratch(offsets, nmocks);
+}
+
+int hp_segment_sum(const double* values, const int64_t* offsets, int32_t nmocks, double* out, double* scratch,
+                   int64_t scratch_elems, void* stream) {
+    if (!offsets || !out || nmocks < 1) return fail("bad arguments");
+    if (!values && offsets[nmocks] > offsets[0]) return fail("values must not be null");
+    for (int m = 0; m < nmocks; m++)
+        if (offsets[m + 1] < offsets[m]) return fail("offsets must be non-decreasing");
+    if (!scratch || scratch_elems < segment_sum_scratch(offsets, nmocks)) return fail("scratch too small");
+    if (launch_segment_sum(values, offsets, nmocks, out, scratch, static_cast<hipStream_t>(stream)))
+        return fail("segment sum launch failed");
     g_err.clear();
     return 0;
 }

@@ -122,6 +122,9 @@ enum Mode : int { kModeCount3D = 0, kModeProj = 1, kModeXYZ = 2, kModeWSum = 3, 
 int launch_pairs(const PairLaunch& L, hipStream_t s);
 
 // reduce.hip
+int64_t segment_sum_scratch(const int64_t* offsets, int nmocks);
+int launch_segment_sum(const double* v, const int64_t* offsets, int nmocks, double* out, double* scratch,
+                       hipStream_t s);
 int launch_rows_dot(const double* rows, int nr, const double* gsuffix, const int64_t* offsets,
                     int nmocks, double* grad, hipStream_t s);
 int launch_add_rows(const double* a, const double* b, double* out, int64_t n, hipStream_t s);

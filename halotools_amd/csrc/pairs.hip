@@ -379,6 +379,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     int qlen = 0;
     int rimg = -1;
     uint32_t rinc = 1u;  // count increment of the ring's entries
+    bool rcorr = false;  // Jacobian auto-count in a shifted image: also the reverse orientation
     double rshx = 0.0, rshy = 0.0, rshz = 0.0;
     int rsxs = 0, rsys = 0, rszs = 0;
     double rax = px, ray = py, raz = pz;
@@ -413,6 +414,29 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                 double dx, dy, dz;
                 sep(bx[u], by[u], bz[u], rshx, rshy, rshz, rsxs, rsys, rszs, rax, ray, raz, dx, dy, dz);
                 a[u] = (u < nv) ? addr_of(dx, dy, dz) : trash_r;
+            }
+        }
+        if (MODE == kModeJac && !SWAP && rcorr) {
+            // Auto-count with one set of weights: the sample-2 rows need the
+            // swapped orientation of each pair, (x2 + x2shift) - x1, which
+            // across the periodic seam may round differently from the forward
+            // one; record the rare disagreements so rows2 = rows1 + corr exactly.
+            const uint32_t* __restrict__ SP = L.strm.perm;
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                if (u >= nv) break;
+                const double rx = rsxs ? (bx[u] + rshx) - px : px - bx[u];
+                const double ry = rsys ? (by[u] + rshy) - py : py - by[u];
+                const double rz = rszs ? (bz[u] + rshz) - pz : pz - bz[u];
+                const uint32_t ar = addr_of(rx, ry, rz);
+                if (ar != a[u]) {
+                    const uint32_t sf = a[u] / step_r, sr = ar / step_r;
+                    double* crow = L.corr + (int64_t)sload_u32_at(SP, sbase + (off[u] >> 3)) * Kr;
+                    if (sf < (uint32_t)Kr)
+                        __hip_atomic_fetch_add(crow + sf, lw[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (sr < (uint32_t)Kr)
+                        __hip_atomic_fetch_add(crow + sr, -lw[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
         }
 #pragma unroll
@@ -478,37 +502,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                     if (sys) ay = py - shy;
                     if (szs) az = pz - shz;
                 }
-                int j = jb;
-                if (MODE == kModeJac && !SWAP && L.corr && (sxs || sys || szs)) {
-                    // Auto-count with one set of weights: the swapped orientation
-                    // of this pair, computed as (x2 + x2shift) - x1, is what the
-                    // sample-2 rows need.  Across the periodic seam the two
-                    // orientations may round apart; record the rare disagreement
-                    // as a correction so rows2 = rows1 + corr exactly.
-                    const uint32_t* __restrict__ SP = L.strm.perm;
-                    evals += (uint64_t)cnt * (uint64_t)(je - jb);
-                    for (; j < je; j++) {
-                        const double qx = SX[j], qy = SY[j], qz = SZ[j];
-                        double dx, dy, dz;
-                        sep(qx, qy, qz, shx, shy, shz, sxs, sys, szs, ax, ay, az, dx, dy, dz);
-                        const uint32_t a = addr_of(dx, dy, dz);
-                        const double rx = sxs ? (qx + shx) - px : dx;
-                        const double ry = sys ? (qy + shy) - py : dy;
-                        const double rz = szs ? (qz + shz) - pz : dz;
-                        const uint32_t ar = addr_of(rx, ry, rz);
-                        accumulate(a, (uint32_t)(j - sbase), 1u);
-                        if (ar != a) {
-                            const uint32_t sf = a / step_r, sr = ar / step_r;
-                            double* crow = L.corr + (int64_t)SP[j] * Kr;
-                            if (sf < (uint32_t)Kr)
-                                __hip_atomic_fetch_add(crow + sf, lw[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            if (sr < (uint32_t)Kr)
-                                __hip_atomic_fetch_add(crow + sr, -lw[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        }
-                    }
-                }
-                if (j >= je) continue;
-
+                const int j = jb;
                 // Point filter: 64 stream points at a time (one per lane, one
                 // vector load) are tested against the group's box grown by the
                 // search length; survivors go to the ring and are evaluated
@@ -524,6 +518,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                     rsxs = sxs; rsys = sys; rszs = szs;
                     rax = ax; ray = ay; raz = az;
                     rinc = (sym && img == 13) ? 2u : 1u;
+                    rcorr = MODE == kModeJac && !SWAP && L.corr != nullptr && img != 13;
                 }
                 const bool after_group = sym && img == 13;
                 const float ftx = (float)(tx * Lx), fty = (float)(ty * Ly), ftz = (float)(tz * Lz);

@@ -175,6 +175,84 @@ int launch_rows_dot(const double* rows, int nr, const double* gsuffix, const int
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// ---------------------------------------------------------------- segment sums
+// out[m] = sum of values[off[m] .. off[m+1]) for every mock m, in a fixed
+// order: stage 1 sums chunks of kSegChunk values (one block each: strided
+// per-thread sums, then a tree), stage 2 adds each mock's chunk sums in
+// chunk order with one wavefront.  No atomics, so results are bitwise
+// reproducible.
+constexpr int kSegMocks = 120;
+constexpr int kSegChunk = 4096;
+struct SegArgs {
+    const double* v;
+    double* part;     // chunk sums of this launch
+    double* out;
+    int nm;           // mocks in this launch
+    int m0;
+    int64_t off[kSegMocks + 1];   // point offsets
+    int64_t cc[kSegMocks + 1];    // cumulative chunk counts (cc[0] = 0)
+};
+
+__global__ void __launch_bounds__(256) k_seg_chunks(const SegArgs a) {
+    __shared__ double t[256];
+    const int64_t c = blockIdx.x;
+    int lo = 0, hi = a.nm - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.cc[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    const int64_t b = a.off[lo] + (c - a.cc[lo]) * kSegChunk;
+    const int64_t e = min(b + (int64_t)kSegChunk, a.off[lo + 1]);
+    double acc = 0.0;
+    for (int64_t i = b + threadIdx.x; i < e; i += 256) acc += a.v[i];
+    t[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) t[threadIdx.x] += t[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.part[c] = t[0];
+}
+
+__global__ void __launch_bounds__(64) k_seg_mocks(const SegArgs a) {
+    const int m = blockIdx.x;
+    const int lane = threadIdx.x;
+    double acc = 0.0;
+    for (int64_t c = a.cc[m] + lane; c < a.cc[m + 1]; c += 64) acc += a.part[c];
+    for (int w = 32; w > 0; w >>= 1) acc += __shfl_xor(acc, w, 64);
+    if (lane == 0) a.out[a.m0 + m] = acc;
+}
+
+int64_t segment_sum_scratch(const int64_t* offsets, int nmocks) {
+    int64_t need = 1;
+    for (int m0 = 0; m0 < nmocks; m0 += kSegMocks) {
+        const int nm = nmocks - m0 < kSegMocks ? nmocks - m0 : kSegMocks;
+        int64_t chunks = 0;
+        for (int i = 0; i < nm; i++) chunks += (offsets[m0 + i + 1] - offsets[m0 + i] + kSegChunk - 1) / kSegChunk;
+        need = chunks > need ? chunks : need;
+    }
+    return need;
+}
+
+int launch_segment_sum(const double* v, const int64_t* offsets, int nmocks, double* out, double* scratch,
+                       hipStream_t s) {
+    for (int m0 = 0; m0 < nmocks; m0 += kSegMocks) {
+        SegArgs a;
+        a.v = v;
+        a.part = scratch;
+        a.out = out;
+        a.nm = nmocks - m0 < kSegMocks ? nmocks - m0 : kSegMocks;
+        a.m0 = m0;
+        a.cc[0] = 0;
+        for (int i = 0; i <= a.nm; i++) a.off[i] = offsets[m0 + i];
+        for (int i = 0; i < a.nm; i++) a.cc[i + 1] = a.cc[i] + (a.off[i + 1] - a.off[i] + kSegChunk - 1) / kSegChunk;
+        if (a.cc[a.nm] > 0)
+            hipLaunchKernelGGL(k_seg_chunks, dim3((unsigned)a.cc[a.nm]), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_seg_mocks, dim3((unsigned)a.nm), dim3(64), 0, s, a);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 __global__ void k_add_rows(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out,
                            int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)

@@ -146,6 +146,16 @@ int hp_marked_npairs_3d(const hp_request* req);
 int hp_marked_backward(const double* rows, int32_t nr, const double* gsuffix,
                        const int64_t* offsets, int32_t nmocks, double* grad, void* stream);
 
+/* Per-mock sums (no counterpart in the reference; the batched model-fitting
+ * step folds per-point gradients into per-mock parameters with it):
+ * out[m] = sum of values[offsets[m] .. offsets[m+1]) in a fixed order (no
+ * atomics: bitwise reproducible).  `offsets` is host memory; values, out and
+ * scratch are device arrays; scratch holds hp_segment_sum_scratch(offsets,
+ * nmocks) doubles (< 0 on error). */
+int64_t hp_segment_sum_scratch(const int64_t* offsets, int32_t nmocks);
+int hp_segment_sum(const double* values, const int64_t* offsets, int32_t nmocks, double* out,
+                   double* scratch, int64_t scratch_elems, void* stream);
+
 /* Diagnostics (host only, no GPU needed; used by the CPU test-suite). */
 /* The kernels' bucketed bin lookup: out[i] = number of suffix-minimum edges of r2 below d[i];
  * info = {shift, base, table entries, multi}. */

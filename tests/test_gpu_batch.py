@@ -5,6 +5,8 @@ marked sums and gradients to 1e-12 relative.  A captured graph is replayed
 several times with new parameters and eager work in between, and every replay
 is checked against an eager step.
 """
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -111,3 +113,20 @@ def test_autotune_keeps_results(data, gpu, tmp_path, monkeypatch):
     step = HODBatchStep(b, torch.cat(lm), logMmin, 0.3, c)
     counts = step.run()[0]
     assert torch.equal(counts, before)
+
+
+def test_segment_sum_fixed_order(gpu):
+    from halotools_amd import _device as D
+    rng = np.random.default_rng(3)
+    sizes = [0, 1, 4095, 4096, 4097, 100000, 0, 3]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    v = torch.tensor(rng.normal(size=int(offs[-1])), device=gpu)
+    seg = D.SegmentSum(offs, gpu)
+    out = torch.empty(len(sizes), dtype=torch.float64, device=gpu)
+    got = seg(v, out).cpu().numpy().copy()
+    host = v.cpu().numpy()
+    for m in range(len(sizes)):
+        ref = math.fsum(host[offs[m]:offs[m + 1]])
+        assert abs(got[m] - ref) <= 1e-13 * max(1.0, np.abs(host[offs[m]:offs[m + 1]]).sum()), m
+    again = seg(v, torch.empty_like(out)).cpu().numpy()
+    assert np.array_equal(got, again)

@@ -215,3 +215,14 @@ def test_mesh_dependent_bins_rejected():
     d1, d2 = _loci()
     with pytest.raises(ValueError):
         npairs_3d(d1, d2, np.array([0.1, -0.2]), period=1)
+
+
+def test_segment_sum_scratch_and_validation():
+    lib = E.library()
+    offs = (ctypes.c_int64 * 4)(0, 4096, 4097, 20000)
+    assert lib.hp_segment_sum_scratch(offs, 3) == 1 + 1 + 4  # ceil(n / 4096) chunks per mock
+    bad = (ctypes.c_int64 * 3)(0, 10, 5)
+    assert lib.hp_segment_sum_scratch(bad, 2) < 0
+    assert "non-decreasing" in lib.hp_last_error().decode()
+    out = ctypes.c_double()
+    assert lib.hp_segment_sum(None, offs, 3, ctypes.byref(out), None, 0, None) < 0
