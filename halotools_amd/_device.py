@@ -77,7 +77,7 @@ def launch(res, stream=None):
 
 def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0, wfunc=0,
             want_counts=True, want_sums=False, rows1=False, rows2=False, want_evals=False,
-            also_counts=False, device=None, pair_events=None, build_events=None, tile_waves=0):
+            also_counts=False, device=None, pair_events=None, build_events=None, tile_waves=0, tangent=False):
     """Build a request with all of its output and scratch buffers allocated.
 
     mocks : list of dicts with n1, n2, period (3,), pbc, cell (3,) (cell <= 0: default)
@@ -86,6 +86,7 @@ def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0
     pair_events, build_events : optional (start, end) HIP event handles recorded
         around the pair kernel(s) and around the cell build
     tile_waves : wavefronts per pair-kernel workgroup (1, 2, 4, 8; 0 = library default)
+    tangent : marked id 1 with (w, dw/dtheta) weights: also res.tangents, d(sums)/dtheta
     The returned object keeps every buffer the request points to alive; the
     same request can be launched again (e.g. inside HIP-graph capture) after
     its input buffers are refreshed in place.
@@ -119,6 +120,8 @@ def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0
             flags |= E.HP_NO_SUMS
         if also_counts:
             flags |= E.HP_ALSO_COUNTS
+        if tangent:
+            flags |= E.HP_TANGENT
     req = E.HpRequest()
     req.kind = kind
     req.flags = flags
@@ -151,6 +154,8 @@ def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0
     req.status = res.status.data_ptr()
     res.sums = torch.empty((M, nr), dtype=torch.float64, device=device) if (marked and want_sums) else None
     req.sums = res.sums.data_ptr() if res.sums is not None else None
+    res.tangents = torch.empty((M, nr), dtype=torch.float64, device=device) if (marked and tangent) else None
+    req.tangents = res.tangents.data_ptr() if res.tangents is not None else None
     res.rows1 = torch.empty((n1_tot, nr), dtype=torch.float64, device=device) if rows1 else None
     res.rows2 = torch.empty((n2_tot, nr), dtype=torch.float64, device=device) if rows2 else None
     req.rows1 = res.rows1.data_ptr() if rows1 else None

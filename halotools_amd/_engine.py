@@ -23,6 +23,7 @@ HP_ROWS1 = 2
 HP_ROWS2 = 4
 HP_NO_SUMS = 8
 HP_ALSO_COUNTS = 16
+HP_TANGENT = 32
 
 EXPORTED_SYMBOLS = (
     "hp_last_error", "hp_abi_version", "hp_workspace_size", "hp_npairs_3d",
@@ -70,6 +71,7 @@ class HpRequest(ctypes.Structure):
         ("stream", ctypes.c_void_p),
         ("pair_events", ctypes.c_void_p * 2),
         ("build_events", ctypes.c_void_p * 2),
+        ("tangents", ctypes.c_void_p),
     ]
 
 

@@ -124,15 +124,25 @@ class MockBatch(object):
 
 
 class HODBatchStep(object):
-    """Counts + marked forward + backward for every mock of a batch, graph-capturable.
+    """Counts + marked sums + their gradient for every mock of a batch, graph-capturable.
 
     Weights are Zheng07 central occupations w_i = 0.5 (1 + erf((log10 M_i - logMmin_m) / sigma)),
     restated from halotools/empirical_models/occupation_models/zheng07_components.py:149-153,
     with one logMmin per mock.  The loss of mock m is sum_k c_k W_mk; the step
     produces counts (M, K), W (M, K), loss (M,) and dloss/dlogMmin (M,).
+
+    mode="tangent" (default): one symmetric pair pass carries (w, dw/dlogMmin)
+    per point and accumulates the counts, W and dW/dlogMmin together
+    (forward-mode: one parameter per mock, so no per-point rows are needed).
+    mode="reverse": a count pass, a marked pass writing the per-point Jacobian
+    rows, the hand-written backward contraction and a fixed-order per-mock sum
+    -- the same arithmetic as the autograd path of ``marked_npairs_3d_product``.
     """
 
-    def __init__(self, batch, log_mass, logMmin, sigma, c):
+    def __init__(self, batch, log_mass, logMmin, sigma, c, mode="tangent"):
+        if mode not in ("tangent", "reverse"):
+            raise ValueError("mode must be 'tangent' or 'reverse'")
+        self.mode = mode
         self.b = batch
         dev = batch.device
         self.log_mass = log_mass.to(dev, torch.float64).contiguous()
@@ -140,35 +150,51 @@ class HODBatchStep(object):
         self.sigma = float(sigma)
         self.c = c.to(dev, torch.float64)
         n = int(batch.offsets[-1])
-        self.w = torch.empty((n, 1), dtype=torch.float64, device=dev)
-        self.grad_w1 = torch.empty(n, dtype=torch.float64, device=dev)
-        self.grad_w2 = torch.empty(n, dtype=torch.float64, device=dev)
+        r2 = batch.rbins * batch.rbins
         self.grad_logMmin = torch.zeros(batch.M, dtype=torch.float64, device=dev)
-        self.G = suffix_sums(self.c).expand(batch.M, -1).contiguous()
-        # counts from the (symmetric) count pass; the marked pass keeps only
-        # its FP64 histograms in LDS, which leaves room for more waves per CU
-        self.res_counts = D.prepare(E.HP_NPAIRS_3D, batch.mocks, batch.cols, batch.cols,
-                                    batch.rbins * batch.rbins, same=True, device=dev)
-        self.res = D.prepare(E.HP_MARKED_NPAIRS_3D, batch.mocks, batch.cols, batch.cols,
-                             batch.rbins * batch.rbins, same=True, w1=self.w, nw=1, wfunc=1,
-                             want_counts=False, want_sums=True, rows1=True, rows2=True,
-                             device=dev, tile_waves=batch.tile_waves)
-        self.dtheta = torch.empty(n, dtype=torch.float64, device=dev)
-        self.segsum = D.SegmentSum(batch.offsets, dev)
+        self.loss = torch.zeros(batch.M, dtype=torch.float64, device=dev)
+        if mode == "tangent":
+            self.wt = torch.empty((n, 2), dtype=torch.float64, device=dev)
+            self.res = D.prepare(E.HP_MARKED_NPAIRS_3D, batch.mocks, batch.cols, batch.cols, r2, same=True,
+                                 w1=self.wt, nw=2, wfunc=1, want_counts=False, want_sums=True, also_counts=True,
+                                 tangent=True, device=dev, tile_waves=batch.tile_waves)
+            self.res_counts = self.res
+        else:
+            self.w = torch.empty((n, 1), dtype=torch.float64, device=dev)
+            self.grad_w1 = torch.empty(n, dtype=torch.float64, device=dev)
+            self.grad_w2 = torch.empty(n, dtype=torch.float64, device=dev)
+            self.G = suffix_sums(self.c).expand(batch.M, -1).contiguous()
+            # counts from the (symmetric) count pass; the marked pass keeps only
+            # its FP64 histograms in LDS, which leaves room for more waves per CU
+            self.res_counts = D.prepare(E.HP_NPAIRS_3D, batch.mocks, batch.cols, batch.cols, r2, same=True,
+                                        device=dev)
+            self.res = D.prepare(E.HP_MARKED_NPAIRS_3D, batch.mocks, batch.cols, batch.cols, r2, same=True,
+                                 w1=self.w, nw=1, wfunc=1, want_counts=False, want_sums=True, rows1=True,
+                                 rows2=True, device=dev, tile_waves=batch.tile_waves)
+            self.dtheta = torch.empty(n, dtype=torch.float64, device=dev)
+            self.segsum = D.SegmentSum(batch.offsets, dev)
         self.graph = None
 
     def _body(self):
         b = self.b
         u = (self.log_mass - self.logMmin[b.mock_id]) / self.sigma
-        self.w[:, 0].copy_(0.5 * (1.0 + torch.erf(u)))
-        D.launch(self.res_counts)
-        D.launch(self.res)
-        D.backward_rows(self.res.rows1, self.G, b.offsets, out=self.grad_w1)
-        D.backward_rows(self.res.rows2, self.G, b.offsets, out=self.grad_w2)
-        dw = -torch.exp(-u * u) / (math.sqrt(math.pi) * self.sigma)
-        torch.mul(self.grad_w1 + self.grad_w2, dw, out=self.dtheta)
-        self.segsum(self.dtheta, self.grad_logMmin)
-        self.loss = (self.res.sums * self.c).sum(dim=1)
+        w = 0.5 * (1.0 + torch.erf(u))
+        dw = -torch.exp(-u * u) / (math.sqrt(math.pi) * self.sigma)   # dw / dlogMmin
+        if self.mode == "tangent":
+            self.wt[:, 0].copy_(w)
+            self.wt[:, 1].copy_(dw)
+            D.launch(self.res)
+            torch.sum(self.res.tangents * self.c, dim=1, out=self.grad_logMmin)
+        else:
+            self.w[:, 0].copy_(w)
+            D.launch(self.res_counts)
+            D.launch(self.res)
+            D.backward_rows(self.res.rows1, self.G, b.offsets, out=self.grad_w1)
+            D.backward_rows(self.res.rows2, self.G, b.offsets, out=self.grad_w2)
+            torch.mul(self.grad_w1 + self.grad_w2, dw, out=self.dtheta)
+            self.segsum(self.dtheta, self.grad_logMmin)
+        # outputs are written in place, so replays and eager steps share them
+        torch.sum(self.res.sums * self.c, dim=1, out=self.loss)
 
     def run(self):
         """One eager step; returns (counts, W, loss, dloss/dlogMmin)."""

@@ -147,7 +147,9 @@ struct Plan {
     size_t o_pcount = 0, o_psum = 0, o_pevals = 0, o_red = 0, o_status = 0;
     bool elim = false;
     size_t o_corr = 0, o_rows_tmp = 0;
-    bool sym = false;  // unweighted auto-count: symmetric evaluation
+    bool sym = false;  // auto-count (unweighted or tangent): symmetric evaluation
+    bool tangent = false;
+    size_t o_ptan = 0;
     size_t total = 0;
     std::vector<unsigned char> consts;  // bytes [0, o_gm[0])
 };
@@ -176,6 +178,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     P.rows2 = P.marked && (r->flags & HP_ROWS2);
     P.sums = P.marked && !(r->flags & HP_NO_SUMS);
     P.counts = !P.marked || (r->flags & HP_ALSO_COUNTS);
+    P.tangent = P.marked && (r->flags & HP_TANGENT);
     P.nw = P.marked ? r->nw : 0;
     if (r->tile_waves != 0 && r->tile_waves != 1 && r->tile_waves != 2 && r->tile_waves != 4 && r->tile_waves != 8)
         return fail("tile_waves must be 0, 1, 2, 4 or 8");
@@ -184,6 +187,8 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
         if (P.nw < 1 || P.nw > 5) return fail("weights per point must be in 1..5");
         if ((P.rows1 || P.rows2) && (r->wfunc != 1 || P.nw != 1))
             return fail("per-point rows (HP_ROWS1/2) need weight_func_id 1 with one weight per point");
+        if (P.tangent && (r->wfunc != 1 || P.nw != 2 || P.rows1 || P.rows2))
+            return fail("HP_TANGENT needs weight_func_id 1 with (w, dw/dtheta) per point and no rows");
     }
     P.Kr = r->nr;
     if (P.Kr < 1 || P.Kr > HP_MAX_BINS) return fail("number of radial bins out of range");
@@ -355,6 +360,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     const int64_t gA = std::max<int64_t>(P.ngroups[0], 1);
     if (P.counts) P.o_pcount = take((size_t)gA * (size_t)P.nb_count * 8);
     if (P.sums) P.o_psum = take((size_t)gA * (size_t)P.Kr * 8);
+    if (P.tangent) P.o_ptan = take((size_t)gA * (size_t)P.Kr * 8);
     P.o_pevals = take((size_t)gA * 8);
     const int64_t red = reduce_scratch_elems(std::max<int64_t>(P.max_groups[0], 1), P.M,
                                              std::max(P.nb_count, P.Kr));
@@ -367,7 +373,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     }
     // unweighted auto-counts evaluate each unshifted unordered pair once; the
     // Hilbert ranks reuse the sort scratch (free once both sides are built)
-    P.sym = P.same && !P.marked;
+    P.sym = P.same && (!P.marked || P.tangent);
     P.total = align_up(off, 256);
 
     // ---- constant blob (geometry, edges, tables) that goes to the workspace
@@ -418,6 +424,7 @@ static int run(const hp_request* r, int kind) {
         return fail("weights must not be null");
     if (P.counts && !r->counts) return fail("counts output must not be null");
     if (P.sums && !r->sums) return fail("sums output must not be null");
+    if (P.tangent && !r->tangents) return fail("tangents output must not be null");
     if (P.rows1 && !r->rows1) return fail("rows1 output must not be null");
     if (P.rows2 && !r->rows2) return fail("rows2 output must not be null");
     hipStream_t s = static_cast<hipStream_t>(r->stream);
@@ -483,13 +490,15 @@ static int run(const hp_request* r, int kind) {
     L.hrank = hrank;
     L.pcount = P.counts ? reinterpret_cast<uint64_t*>(ws + P.o_pcount) : nullptr;
     L.psum = P.sums ? reinterpret_cast<double*>(ws + P.o_psum) : nullptr;
+    L.ptan = P.tangent ? reinterpret_cast<double*>(ws + P.o_ptan) : nullptr;
     L.pevals = reinterpret_cast<uint64_t*>(ws + P.o_pevals);
     switch (kind) {
         case HP_NPAIRS_3D: L.mode = kModeCount3D; break;
         case HP_NPAIRS_PROJECTED: L.mode = kModeProj; break;
         case HP_NPAIRS_XY_Z: L.mode = kModeXYZ; break;
         default:
-            L.mode = (r->wfunc == 1 && P.nw == 1 && (P.rows1 || P.rows2)) ? kModeJac : kModeWSum;
+            L.mode = P.tangent ? kModeTan
+                               : ((r->wfunc == 1 && P.nw == 1 && (P.rows1 || P.rows2)) ? kModeJac : kModeWSum);
             L.also_count = P.counts ? 1 : 0;
             L.rows = P.rows1 ? r->rows1 : nullptr;
             break;
@@ -541,6 +550,11 @@ static int run(const hp_request* r, int kind) {
     if (P.sums) {
         if (launch_reduce_sums_s(L.psum, P.Kr, dgeom, P.M, 0, P.max_groups[0], reinterpret_cast<double*>(red),
                                  r->sums, s))
+            return fail("reduction launch failed");
+    }
+    if (P.tangent) {
+        if (launch_reduce_sums_s(L.ptan, P.Kr, dgeom, P.M, 0, P.max_groups[0], reinterpret_cast<double*>(red),
+                                 r->tangents, s))
             return fail("reduction launch failed");
     }
     if (r->evals) {

@@ -111,6 +111,7 @@ struct PairLaunch {
     const void* ws;        // workspace base (edges / tables)
     uint64_t* pcount;      // [ngroups][nbins_count]
     double* psum;          // [ngroups][nr]
+    double* ptan;          // tangent mode: [ngroups][nr]
     double* rows;          // [n_lane_total][nr] (original order) or nullptr
     double* corr;          // auto-counts: rows2 - rows1 corrections from seam pairs, or nullptr
     uint64_t* pevals;      // [ngroups]
@@ -118,7 +119,7 @@ struct PairLaunch {
     // unshifted unordered pair is then evaluated once and counted twice
     const uint32_t* hrank;
 };
-enum Mode : int { kModeCount3D = 0, kModeProj = 1, kModeXYZ = 2, kModeWSum = 3, kModeJac = 4 };
+enum Mode : int { kModeCount3D = 0, kModeProj = 1, kModeXYZ = 2, kModeWSum = 3, kModeJac = 4, kModeTan = 5 };
 int launch_pairs(const PairLaunch& L, hipStream_t s);
 
 // reduce.hip

@@ -22,6 +22,8 @@
 // with FP contraction off, and binned with the same "<=" semantics.  A pair
 // within the search radius is visited exactly once with its minimum image;
 // other images of it lie at least L - rmax >= 2 rmax away and bin nowhere.
+#include <algorithm>
+
 #include "hp_internal.h"
 
 #pragma clang fp contract(off)
@@ -159,9 +161,11 @@ struct PairArgs {
     PairLaunch L;
     int wave_bytes;     // histogram bytes per wave
     int cnt_off;        // marked + counts: byte offset of the count rows
+    int tan_off;        // tangent mode: byte offset of the second FP64 rows
     int shared_count;   // 1: one histogram per wave instead of per lane
     int smem;           // dynamic LDS bytes
     int ring_off;       // byte offset of the survivor ring within a wave's LDS
+    int tab_bytes;      // LDS bytes of the bucket tables, ahead of the wave areas
 };
 
 // Survivor ring per wave: up to B-1 leftover entries plus one 64-point chunk.
@@ -170,14 +174,17 @@ constexpr int kRing = 72;
 // Occupancy the register allocation is held to (waves per SIMD): the count
 // kernels are limited by VGPRs otherwise, the FP64-histogram kernels by LDS.
 template <int MODE>
-constexpr int kWavesPerSimd = (MODE == kModeWSum || MODE == kModeJac) ? 1 : (MODE == kModeXYZ ? 5 : 6);
+constexpr int kWavesPerSimd = (MODE == kModeJac) ? 1 : ((MODE == kModeWSum || MODE == kModeTan) ? 4
+                                                                                 : (MODE == kModeXYZ ? 5 : 6));
 
 template <int MODE, int SWAP, int ALSO, int MULTI, int WAVES>
 __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const PairArgs A) {
-    constexpr bool F64 = (MODE == kModeWSum || MODE == kModeJac);
-    __shared__ LdsEntry s_tr[kTableMax];
-    __shared__ LdsEntry s_tp[MODE == kModeXYZ ? kTableMax : 1];
-    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr bool F64 = (MODE == kModeWSum || MODE == kModeJac || MODE == kModeTan);
+    // LDS: the bucket tables (sized to this request's buckets), then one area per wave
+    extern __shared__ __align__(16) unsigned char lds[];
+    LdsEntry* const s_tr = reinterpret_cast<LdsEntry*>(lds);
+    LdsEntry* const s_tp = s_tr + A.L.br.nb;
+    unsigned char* const smem = lds + A.tab_bytes;
     const PairLaunch& L = A.L;
     const int Kr = L.br.nbins, Kp = L.bp.nbins;
     const int lane = threadIdx.x & 63;
@@ -185,9 +192,19 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
 
     // histogram geometry: rows of one bin; (Kr+1) rows, the last one a trash
     // row for pairs beyond the largest edge (no branch in the pair loop)
+    // Sums that need no per-lane factor (generic marks, tangent sums) share
+    // FP64 rows among lanes l, l+16, l+32, l+48: an 8-byte LDS operation is
+    // served 16 lanes per pass, so lanes of one column never collide within
+    // a pass, and the rows take a quarter of the LDS (more waves per CU).
+    // Their count rows are shared by lanes l and l+32 the same way.
+    constexpr bool SHARE4 = (MODE == kModeWSum || MODE == kModeTan);
     const bool shared = (MODE == kModeXYZ) || (!F64 && A.shared_count);
-    const uint32_t step_r = (MODE == kModeXYZ) ? (uint32_t)(Kp + 1) * 4u : (shared ? 4u : (F64 ? 512u : 256u));
-    const uint32_t lane_off = shared ? 0u : (uint32_t)lane * (F64 ? 8u : 4u);
+    const uint32_t step_r = (MODE == kModeXYZ)
+                                ? (uint32_t)(Kp + 1) * 4u
+                                : (shared ? 4u : (SHARE4 ? 128u : (F64 ? 512u : 256u)));
+    const uint32_t lane_off =
+        shared ? 0u : (SHARE4 ? (uint32_t)(lane & 15) * 8u : (uint32_t)lane * (F64 ? 8u : 4u));
+    const uint32_t cnt_col = SHARE4 ? (uint32_t)(lane & 31) * 4u : 0u;  // SHARE4 count rows
     {
         const char* ws = static_cast<const char*>(L.ws);
         const TabEntry* gtr = reinterpret_cast<const TabEntry*>(ws + L.br.table_off);
@@ -346,11 +363,11 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     // points are paired separately below; shifted images (whose two orders
     // may round apart) are evaluated from both sides, once each.
     const uint32_t* __restrict__ HR = L.hrank;
-    const bool sym = !F64 && !SWAP && HR != nullptr;
+    const bool sym = (!F64 || MODE == kModeTan) && !SWAP && HR != nullptr;
     const uint32_t hend = (uint32_t)(p0 + cnt);
     // r: mock-relative stream index
     // inc: count-mode increment (2 for an unshifted pair evaluated once for both orders)
-    auto accumulate = [&](uint32_t a, uint32_t r, uint32_t inc) __attribute__((always_inline)) {
+    auto accumulate = [&](uint32_t a, uint32_t r, uint32_t inc, bool) __attribute__((always_inline)) {
         if (!F64) {
             __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(hl + a), inc, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -366,8 +383,9 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
             __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a), v, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
             if (ALSO) {
-                __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(wsm + A.cnt_off + ((a + lane_off) >> 1)), 1u,
-                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                const uint32_t ca = SHARE4 ? a + cnt_col : ((a + lane_off) >> 1);
+                __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(wsm + A.cnt_off + ca), 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
     };
@@ -380,6 +398,9 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     int rimg = -1;
     uint32_t rinc = 1u;  // count increment of the ring's entries
     bool rcorr = false;  // Jacobian auto-count in a shifted image: also the reverse orientation
+    bool rhalf = false;  // tangent mode under symmetry, shifted image
+    // tangent mode: the lane's (w, t), halved for a shifted image under symmetry
+    double rl0 = lw[0], rl1 = lw[1];
     double rshx = 0.0, rshy = 0.0, rshz = 0.0;
     int rsxs = 0, rsys = 0, rszs = 0;
     double rax = px, ray = py, raz = pz;
@@ -439,8 +460,46 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                 }
             }
         }
+        if (MODE == kModeTan || MODE == kModeJac) {
+            // weights of the whole batch in one scalar-load wait; most
+            // evaluated pairs lie beyond the largest edge, and their lanes skip
+            // the FP64 atomics instead of adding to the trash row
+            double bw[B], bt[B];
 #pragma unroll
-        for (int u = 0; u < B; u++) accumulate(a[u], off[u] >> 3, rinc);
+            for (int u = 0; u < B; u++) {
+                if (MODE == kModeTan) {
+                    bw[u] = sload_f64(SWm, off[u] * 2u);
+                    bt[u] = sload_f64(SWm, off[u] * 2u + 8u);
+                } else {
+                    bw[u] = sload_f64(SWm, off[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                if (a[u] == trash_r) continue;
+                if (MODE == kModeTan) {
+                    // pair terms w_i w_j and t_i w_j + w_i t_j (t = dw/dtheta)
+                    const double v = rl0 * bw[u];
+                    const double tv = rl1 * bw[u] + rl0 * bt[u];
+                    __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a[u]), v, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + A.tan_off + a[u]), tv, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (ALSO)
+                        __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(wsm + A.cnt_off + a[u] + cnt_col), rinc,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+                    __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a[u]), bw[u], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (ALSO)
+                        __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(wsm + A.cnt_off + ((a[u] + lane_off) >> 1)),
+                                               1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < B; u++) accumulate(a[u], off[u] >> 3, rinc, false);
+        }
     };
     auto drain = [&]() __attribute__((always_inline)) {
         for (int q = 0; q < qlen; q += B) consume(q, min(B, qlen - q));
@@ -519,6 +578,11 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                     rax = ax; ray = ay; raz = az;
                     rinc = (sym && img == 13) ? 2u : 1u;
                     rcorr = MODE == kModeJac && !SWAP && L.corr != nullptr && img != 13;
+                    rhalf = MODE == kModeTan && sym && img != 13;
+                    if (MODE == kModeTan) {
+                        rl0 = rhalf ? 0.5 * lw[0] : lw[0];
+                        rl1 = rhalf ? 0.5 * lw[1] : lw[1];
+                    }
                 }
                 const bool after_group = sym && img == 13;
                 const float ftx = (float)(tx * Lx), fty = (float)(ty * Ly), ftz = (float)(tz * Lz);
@@ -568,13 +632,17 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
         // point k counts 2 when k follows i in the lane order, 1 for k == i
         // (the self pair), 0 when k precedes i (counted from the other side)
         for (int k0 = 0; k0 < cnt; k0 += B) {
-            double bx[B], by[B], bz[B];
+            double bx[B], by[B], bz[B], bw[B], bt[B];
 #pragma unroll
             for (int u = 0; u < B; u++) {
                 const int64_t qk = (int64_t)sload_u32_at(L.lane.perm, p0 + min(k0 + u, cnt - 1));
                 bx[u] = sload_f64_at(L.lane.x, qk);
                 by[u] = sload_f64_at(L.lane.y, qk);
                 bz[u] = sload_f64_at(L.lane.z, qk);
+                if (MODE == kModeTan) {
+                    bw[u] = sload_f64_at(L.lane.w, 2 * qk);
+                    bt[u] = sload_f64_at(L.lane.w, 2 * qk + 1);
+                }
             }
             evals += (uint64_t)cnt * (uint64_t)min(B, cnt - k0);
             uint32_t a[B];
@@ -586,7 +654,26 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                 a[u] = (k < cnt && k >= lane) ? addr_of(dx, dy, dz) : trash_r;
             }
 #pragma unroll
-            for (int u = 0; u < B; u++) accumulate(a[u], 0u, (k0 + u > lane) ? 2u : 1u);
+            for (int u = 0; u < B; u++) {
+                const uint32_t inc = (k0 + u > lane) ? 2u : 1u;
+                if (MODE == kModeTan && a[u] != trash_r) {
+                    // rows are doubled at the end: the self pair adds half
+                    const double h = (k0 + u > lane) ? 1.0 : 0.5;
+                    const double wj = bw[u] * h, tj = bt[u] * h;
+                    const double v = lw[0] * wj;
+                    const double tv = lw[1] * wj + lw[0] * tj;
+                    __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a[u]), v, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + A.tan_off + a[u]), tv, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (ALSO) {
+                        __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(wsm + A.cnt_off + a[u] + cnt_col), inc,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                } else if (MODE != kModeTan) {
+                    accumulate(a[u], 0u, inc, false);
+                }
+            }
         }
     }
 
@@ -599,7 +686,8 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     } else if (MODE == kModeCount3D || MODE == kModeProj || ALSO) {
         uint64_t* out = L.pcount + g * Kr;
         for (int s = 0; s < Kr; s++) {
-            const uint64_t t = wave_sum_u64((uint64_t)ch32[s * 64 + lane]);
+            const uint32_t c = SHARE4 ? (lane < 32 ? ch32[s * 32 + lane] : 0u) : ch32[s * 64 + lane];
+            const uint64_t t = wave_sum_u64((uint64_t)c);
             if (lane == 0) out[s] = t;
         }
     }
@@ -607,8 +695,21 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     if (MODE == kModeWSum) {
         if (L.psum) {
             for (int s = 0; s < Kr; s++) {
-                const double t = wave_sum_f64(fh[s * 64 + lane]);
+                const double t = wave_sum_f64(lane < 16 ? fh[s * 16 + lane] : 0.0);
                 if (lane == 0) L.psum[g * Kr + s] = t;
+            }
+        }
+    }
+    if (MODE == kModeTan) {
+        // rows of w_i w_j and of t_i w_j + w_i t_j, doubled under symmetry
+        const double* th = reinterpret_cast<const double*>(wsm + A.tan_off);
+        const double sc = sym ? 2.0 : 1.0;
+        for (int s = 0; s < Kr; s++) {
+            const double ws = wave_sum_f64(lane < 16 ? fh[s * 16 + lane] : 0.0);
+            const double ts = wave_sum_f64(lane < 16 ? th[s * 16 + lane] : 0.0);
+            if (lane == 0) {
+                if (L.psum) L.psum[g * Kr + s] = sc * ws;
+                L.ptan[g * Kr + s] = sc * ts;
             }
         }
     }
@@ -658,6 +759,7 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
     const int Kr = L.br.nbins, Kp = L.bp.nbins;
     A.shared_count = 0;
     A.cnt_off = 0;
+    A.tan_off = 0;
     int wave_bytes = 0;
     switch (L.mode) {
         case kModeCount3D:
@@ -666,24 +768,56 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
             else wave_bytes = (Kr + 1) * 64 * 4;
             break;
         case kModeXYZ: wave_bytes = (Kr + 1) * (Kp + 1) * 4; break;
-        case kModeWSum:
         case kModeJac:
             A.cnt_off = (Kr + 1) * 64 * 8;
             wave_bytes = A.cnt_off + (L.also_count ? (Kr + 1) * 64 * 4 : 0);
+            break;
+        case kModeWSum:  // rows shared by 4 lanes (16 columns), counts by 2 (32 columns)
+            A.cnt_off = (Kr + 1) * 16 * 8;
+            wave_bytes = A.cnt_off + (L.also_count ? (Kr + 1) * 32 * 4 : 0);
+            break;
+        case kModeTan:
+            A.tan_off = (Kr + 1) * 16 * 8;
+            A.cnt_off = 2 * A.tan_off;
+            wave_bytes = A.cnt_off + (L.also_count ? (Kr + 1) * 32 * 4 : 0);
             break;
         default: return -1;
     }
     if (L.br.nb > kTableMax || L.bp.nb > kTableMax) return -1;
     A.ring_off = align16(wave_bytes);
     A.wave_bytes = align16(A.ring_off + kRing * 4);
-    const int static_lds = (int)sizeof(LdsEntry) * kTableMax * (L.mode == kModeXYZ ? 2 : 1);
-#ifndef HP_MAX_WAVES
-#define HP_MAX_WAVES 8
-#endif
-    int waves = L.waves > 0 ? L.waves : HP_MAX_WAVES;
-    while (waves > 1 && static_lds + waves * A.wave_bytes > 72 * 1024) waves >>= 1;
-    A.smem = waves * A.wave_bytes;
-    if (static_lds + A.smem > 160 * 1024) return -2;
+    A.tab_bytes = align16((int)sizeof(LdsEntry) * (L.br.nb + (L.mode == kModeXYZ ? L.bp.nb : 0)));
+    // Workgroup shape: the number of waves per workgroup that keeps the most
+    // waves resident per CU (160 KB of LDS; register-limited waves per CU as
+    // the kernels' launch bounds hold them); on ties 4 waves, then 8 (4 was
+    // never slower than 8 in measurements of every mode).
+    constexpr int kLdsPerCU = 160 * 1024;
+    static int lds_per_block = 0;  // the device's limit per workgroup (queried once)
+    if (lds_per_block == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            lds_per_block = std::min(v, kLdsPerCU);
+        else
+            lds_per_block = 64 * 1024;
+    }
+    const int kLdsPerBlock = lds_per_block;
+    const int reg_waves = 4 * (L.mode == kModeJac ? 6 : ((L.mode == kModeWSum || L.mode == kModeTan) ? 4
+                                                          : (L.mode == kModeXYZ ? 5 : 6)));
+    int waves = 0, best = -1;
+    if (L.waves > 0) {
+        waves = L.waves;
+    } else {
+        for (const int w : {4, 8, 2, 1}) {
+            const int bytes = A.tab_bytes + w * A.wave_bytes;
+            if (bytes > kLdsPerBlock) continue;
+            const int resident = std::min(w * (kLdsPerCU / bytes), reg_waves);
+            if (resident > best) { best = resident; waves = w; }
+        }
+        if (waves == 0) return -2;
+    }
+    A.smem = A.tab_bytes + waves * A.wave_bytes;
+    if (A.smem > kLdsPerBlock) return -2;
     const int multi = (L.br.multi || L.bp.multi) ? 1 : 0;
     const int also = L.also_count ? 1 : 0;
     switch (L.mode) {
@@ -696,6 +830,8 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
             if (L.swap)
                 return also ? launch_m<kModeJac, 1, 1>(A, waves, multi, s) : launch_m<kModeJac, 1, 0>(A, waves, multi, s);
             return also ? launch_m<kModeJac, 0, 1>(A, waves, multi, s) : launch_m<kModeJac, 0, 0>(A, waves, multi, s);
+        case kModeTan:
+            return also ? launch_m<kModeTan, 0, 1>(A, waves, multi, s) : launch_m<kModeTan, 0, 0>(A, waves, multi, s);
     }
     return -1;
 }

@@ -57,7 +57,10 @@ enum hp_flags {
     HP_ROWS1 = 2,        /* marked id 1: write rows1 = d sums / d weights1 per point   */
     HP_ROWS2 = 4,        /* marked id 1: write rows2 = d sums / d weights2 per point   */
     HP_NO_SUMS = 8,      /* marked: skip the weighted sums (rows only)                */
-    HP_ALSO_COUNTS = 16  /* marked: also produce the unweighted counts of npairs_3d   */
+    HP_ALSO_COUNTS = 16, /* marked: also produce the unweighted counts of npairs_3d   */
+    HP_TANGENT = 32      /* marked, weight_func_id 1 with nw = 2 weights per point    */
+                         /* (w, dw/dtheta): sums of w1*w2 and `tangents`, the sums of */
+                         /* t1*w2 + w1*t2 = d(sums)/dtheta (forward-mode derivative)  */
 };
 
 /* One realisation ("mock") of a batched request.  A single call is nmocks=1. */
@@ -104,6 +107,7 @@ typedef struct hp_request {
                                /* before and after the pair kernel(s) (profiling)      */
     void* build_events[2];     /* optional hipEvent_t pair recorded right before and   */
                                /* after the cell build (profiling)                     */
+    double* tangents;          /* HP_TANGENT: [nmocks][nr] cumulative d(sums)/dtheta    */
 } hp_request;
 
 /* Thread-local message describing the last error (empty string if none). */

@@ -130,3 +130,78 @@ def test_segment_sum_fixed_order(gpu):
         assert abs(got[m] - ref) <= 1e-13 * max(1.0, np.abs(host[offs[m]:offs[m + 1]]).sum()), m
     again = seg(v, torch.empty_like(out)).cpu().numpy()
     assert np.array_equal(got, again)
+
+
+@pytest.mark.parametrize("mode", ["tangent", "reverse"])
+def test_hod_step_modes_agree(data, gpu, mode):
+    from halotools_amd.batch import HODBatchStep, MockBatch, one_at_a_time
+    sizes, pts, lm, logMmin, c = data
+    b = MockBatch(pts, L, RB, device=gpu)
+    counts, W, loss, grad = [t.clone() for t in HODBatchStep(b, torch.cat(lm), logMmin, 0.3, c, mode=mode).run()]
+    ref = one_at_a_time(pts, lm, logMmin, 0.3, RB, c, L)
+    for m in range(len(sizes)):
+        assert np.array_equal(counts[m].cpu().numpy(), ref[0][m].cpu().numpy())
+        np.testing.assert_allclose(W[m].cpu().numpy(), ref[1][m].cpu().numpy(), rtol=1e-12)
+        np.testing.assert_allclose(grad[m].item(), ref[3][m].item(), rtol=1e-10, atol=1e-12)
+
+
+def _tangent_pass(pts1, pts2, w1, t1, w2, t2, rbins, period, gpu):
+    from halotools_amd import _device as D
+    from halotools_amd import _engine as E
+    same = pts2 is None
+    cols = lambda p: tuple(p[:, i].contiguous() for i in range(3))
+    s1 = cols(pts1)
+    s2 = s1 if same else cols(pts2)
+    wt1 = torch.stack([w1, t1], dim=1).contiguous()
+    wt2 = None if same else torch.stack([w2, t2], dim=1).contiguous()
+    n1, n2 = len(pts1), len(pts1) if same else len(pts2)
+    Lb = period if period is not None else 400.0
+    mock = dict(n1=n1, n2=n2, period=[Lb] * 3, pbc=period is not None)
+    res = D.run(E.HP_MARKED_NPAIRS_3D, [mock], s1, s2, rbins * rbins, same=same, w1=wt1, w2=wt2, nw=2, wfunc=1,
+                want_counts=False, want_sums=True, also_counts=True, tangent=True, device=gpu)
+    torch.cuda.synchronize()
+    assert int(res.cbuf[-1].item()) & 0xFFFFFFFF == 0  # int32 status word
+    return res
+
+
+@pytest.mark.parametrize("same,period", [(True, 200.0), (False, 200.0), (True, None)])
+def test_tangent_pass_matches_reverse_mode_and_fd(gpu, same, period):
+    # W and dW/dtheta of the tangent pass against the product-weighted sums,
+    # their derivative through the hand-written backward (reverse mode) and
+    # central finite differences; counts against npairs_3d
+    from halotools_amd.autodiff import marked_npairs_3d_product
+    from halotools_amd.mock_observables import npairs_3d
+    rng = np.random.default_rng([11, int(same), int(period is None)])
+    n1, n2 = 6000, 4500
+    p1 = torch.tensor(rng.uniform(0, 200.0, (n1, 3)), device=gpu)
+    p2 = p1 if same else torch.tensor(rng.uniform(0, 200.0, (n2, 3)), device=gpu)
+    # points on the periodic seam exercise the shifted images
+    p1[:50, 0] = 200.0 - 1e-9 * torch.arange(50, device=gpu)
+    if period is None:
+        # the drop-ins shift non-periodic samples by their smallest coordinate;
+        # starting at 0 makes that shift exact, so both paths see the same inputs
+        m = min(float(p1.min()), float(p2.min()))
+        p1 = p1 - m
+        p2 = p1 if same else p2 - m
+    x1 = torch.tensor(rng.uniform(-1, 1, n1), device=gpu)
+    x2 = x1 if same else torch.tensor(rng.uniform(-1, 1, len(p2)), device=gpu)
+    theta = 0.3
+    w = lambda x, th: torch.sigmoid(x + th)
+    dw = lambda x, th: torch.sigmoid(x + th) * (1 - torch.sigmoid(x + th))
+    rb = np.logspace(-0.5, np.log10(15.0), 12)
+    res = _tangent_pass(p1, None if same else p2, w(x1, theta), dw(x1, theta), w(x2, theta), dw(x2, theta),
+                        rb, period, gpu)
+    th = torch.tensor(theta, dtype=torch.float64, device=gpu, requires_grad=True)
+    W = marked_npairs_3d_product(p1, p2, rb, w(x1, th), w(x2, th), period=period)
+    np.testing.assert_allclose(res.sums[0].cpu().numpy(), W.detach().cpu().numpy(), rtol=1e-12)
+    c = torch.tensor(rng.normal(size=len(rb)), device=gpu)
+    (W * c).sum().backward()
+    dtan = float((res.tangents[0] * c).sum())
+    np.testing.assert_allclose(dtan, th.grad.item(), rtol=1e-10)
+    h = 1e-5
+    wp = marked_npairs_3d_product(p1, p2, rb, w(x1, theta + h), w(x2, theta + h), period=period)
+    wm = marked_npairs_3d_product(p1, p2, rb, w(x1, theta - h), w(x2, theta - h), period=period)
+    fd = ((wp - wm) / (2 * h)).cpu().numpy()
+    np.testing.assert_allclose(res.tangents[0].cpu().numpy(), fd, rtol=1e-6, atol=1e-9)
+    counts = npairs_3d(p1, p2, rb, period=period)
+    assert torch.equal(res.counts[0], counts)
