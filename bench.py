@@ -26,6 +26,10 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector, with FMA (non-FMA issue ceiling is half)
 HBM_PEAK_GBPS = 8000.0
 ALG_BUILD_BYTES = 76
+# HBM bytes per launch of the pair kernel on this workload: rocprofv3
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors being what the same
+# counters read for a plain 256 MiB copy (profiles/k_pairs_count3d_clustered_1M.md)
+PAIR_KERNEL_TRAFFIC_BYTES = 3.926e8
 
 
 def reference_candidates(pts, rmax, L):
@@ -65,8 +69,7 @@ def bench_batched(dev, steps):
     import numpy as np
     import torch
 
-    from halotools_amd import _engine as E
-    from halotools_amd import batch, mocks
+    from halotools_amd import autotune, batch, mocks
     rng = np.random.default_rng([43, 5])
     M, L = 64, 1000.0
     sizes = np.round(np.exp(rng.uniform(np.log(5e4), np.log(2e6), M))).astype(int)
@@ -80,7 +83,7 @@ def bench_batched(dev, steps):
     # autotune: cell size per density bucket and the workgroup shape, timed on
     # the batch's own largest mock of each bucket in the fused counts+Jacobian mode
     t0 = time.perf_counter()
-    tuned = b.tune(E.HP_MARKED_NPAIRS_3D, reps=1)
+    tuned = b.tune(autotune.TANGENT, reps=1)
     t_tune = time.perf_counter() - t0
 
     # cell build at this scale (all 64 mocks in one counting sort): HIP events
@@ -350,7 +353,7 @@ def main():
         # compute-bound: FP64 vector ALU (a distance threshold, not a contraction, so no MFMA);
         # MI355X's FP64 peak is 78.6 TFLOP/s for vector and matrix alike
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None},
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": PAIR_KERNEL_TRAFFIC_BYTES},
         "batched_64_mocks": batched,
         "other_configs": others,
         "cpu_baseline": cpu,

@@ -12,8 +12,9 @@ log2 density bucket, search radius / box bucket).  Results are cached in
 memory and, when HALOTOOLS_AMD_TUNE_CACHE names a file, in that JSON file.
 
 Modes: ``E.HP_NPAIRS_3D`` times npairs_3d auto-counts;
-``E.HP_MARKED_NPAIRS_3D`` times the product-weighted pass that also produces
-counts and both samples' Jacobian rows (the batched model-fitting step).
+``E.HP_MARKED_NPAIRS_3D`` times the product-weighted pass that produces both
+samples' Jacobian rows (reverse mode); ``TANGENT`` times the forward-mode pass
+with counts (the batched model-fitting step).
 """
 import json
 import math
@@ -27,6 +28,7 @@ from . import _device as D
 from . import _engine as E
 
 TARGETS = (8.0, 32.0, 128.0, 4096.0)
+TANGENT = 1000  # tuner key of the marked tangent pass (not a library kind)
 WAVES = (2, 4, 8)
 _CACHE = {}
 
@@ -77,22 +79,27 @@ def _request(kind, n, period, cols, rbins, edge, waves, device):
     r2 = rbins * rbins
     if kind == E.HP_NPAIRS_3D:
         return D.prepare(kind, [mock], cols, cols, r2, same=True, device=device, tile_waves=waves)
+    if kind == TANGENT:
+        wt = torch.ones((n, 2), dtype=torch.float64, device=device)
+        return D.prepare(E.HP_MARKED_NPAIRS_3D, [mock], cols, cols, r2, same=True, w1=wt, nw=2, wfunc=1,
+                         want_counts=False, want_sums=True, also_counts=True, tangent=True, device=device,
+                         tile_waves=waves)
     w = torch.ones((n, 1), dtype=torch.float64, device=device)
     return D.prepare(kind, [mock], cols, cols, r2, same=True, w1=w, nw=1, wfunc=1, want_counts=False,
-                     want_sums=True, rows1=True, rows2=True, also_counts=True, device=device, tile_waves=waves)
+                     want_sums=True, rows1=True, rows2=True, device=device, tile_waves=waves)
 
 
 def tune(points, period, rbins, kind=E.HP_NPAIRS_3D, reps=2, targets=TARGETS, waves=WAVES, device=None):
     """Time auto-counts of `points` for every (target, waves) and cache the fastest.
 
     points : (N, 3) array or CUDA tensor inside the periodic box `period`.
-    kind   : E.HP_NPAIRS_3D or E.HP_MARKED_NPAIRS_3D (see the module docstring).
+    kind   : E.HP_NPAIRS_3D, E.HP_MARKED_NPAIRS_3D or TANGENT (see the module docstring).
     Returns ((best_target, best_waves), {(target, waves): seconds}).  The counts
     of every configuration are checked equal: the cell size and the tile shape
     change the work schedule, never the result.
     """
-    if kind not in (E.HP_NPAIRS_3D, E.HP_MARKED_NPAIRS_3D):
-        raise ValueError("tune: kind must be HP_NPAIRS_3D or HP_MARKED_NPAIRS_3D")
+    if kind not in (E.HP_NPAIRS_3D, E.HP_MARKED_NPAIRS_3D, TANGENT):
+        raise ValueError("tune: kind must be HP_NPAIRS_3D, HP_MARKED_NPAIRS_3D or TANGENT")
     device = D.require_gpu(device)
     t = points if isinstance(points, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(points))
     t = t.to(device=device, dtype=torch.float64)
@@ -116,11 +123,14 @@ def tune(points, period, rbins, kind=E.HP_NPAIRS_3D, reps=2, targets=TARGETS, wa
                 D.launch(res)
                 torch.cuda.synchronize(device)
                 best = min(best, time.perf_counter() - t0)
-            counts = res.counts[0].cpu().numpy()
+            counts = res.counts[0].cpu().numpy() if res.counts is not None else res.sums[0].cpu().numpy()
             if ref is None:
                 ref = counts
             elif not np.array_equal(ref, counts):
-                raise RuntimeError("counts changed with the cell size or tile shape; this is a bug")
+                if res.counts is not None:
+                    raise RuntimeError("counts changed with the cell size or tile shape; this is a bug")
+                if not np.allclose(ref, counts, rtol=1e-12, atol=0):
+                    raise RuntimeError("sums changed with the cell size or tile shape; this is a bug")
             timing[(target, wv)] = best
             del res
     winner = min(timing, key=timing.get)

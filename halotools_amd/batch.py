@@ -77,7 +77,7 @@ class MockBatch(object):
                 if m == heaviest and not keep_waves:
                     self.tile_waves = hit[1]
 
-    def tune(self, kind=E.HP_MARKED_NPAIRS_3D, **kw):
+    def tune(self, kind=autotune.TANGENT, **kw):
         """Autotune cell edges and the tile shape on this batch's own mocks.
 
         One representative (the largest mock) per density bucket of the tuner's

@@ -97,7 +97,8 @@ def test_tile_waves_do_not_change_results(data, gpu, waves):
     assert torch.equal(b.marked_npairs_3d_product(w), ref.marked_npairs_3d_product(w))
 
 
-def test_autotune_keeps_results(data, gpu, tmp_path, monkeypatch):
+@pytest.mark.parametrize("which", ["reverse", "tangent"])
+def test_autotune_keeps_results(data, gpu, tmp_path, monkeypatch, which):
     from halotools_amd import autotune
     from halotools_amd import _engine as E
     from halotools_amd.batch import HODBatchStep, MockBatch
@@ -106,7 +107,8 @@ def test_autotune_keeps_results(data, gpu, tmp_path, monkeypatch):
     sizes, pts, lm, logMmin, c = data
     b = MockBatch(pts, L, RB, device=gpu)
     before = b.npairs_3d().clone()
-    tuned = b.tune(E.HP_MARKED_NPAIRS_3D, reps=1, targets=(2.0, 8.0), waves=(2, 8))
+    kind = E.HP_MARKED_NPAIRS_3D if which == "reverse" else autotune.TANGENT
+    tuned = b.tune(kind, reps=1, targets=(2.0, 8.0), waves=(2, 8))
     assert tuned and (tmp_path / "tune.json").exists()
     assert b.tile_waves in (2, 8)
     assert torch.equal(b.npairs_3d(), before)
