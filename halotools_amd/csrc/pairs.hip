@@ -168,8 +168,9 @@ struct PairArgs {
     int tab_bytes;      // LDS bytes of the bucket tables, ahead of the wave areas
 };
 
-// Survivor ring per wave: up to B-1 leftover entries plus one 64-point chunk.
-constexpr int kRing = 72;
+// Survivor ring per wave: up to kBatch-1 leftover entries plus one 64-point chunk.
+constexpr int kRing = (kBatch - 1 + 64 + 3) & ~3;
+static_assert(kRing >= kBatch - 1 + 64, "survivor ring too small for the batch size");
 
 // Occupancy the register allocation is held to (waves per SIMD): the count
 // kernels are limited by VGPRs otherwise, the FP64-histogram kernels by LDS.
@@ -197,13 +198,17 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     // served 16 lanes per pass, so lanes of one column never collide within
     // a pass, and the rows take a quarter of the LDS (more waves per CU).
     // Their count rows are shared by lanes l and l+32 the same way.
+    // The count rows of the count modes are shared by lanes l and l+32 (a
+    // 4-byte LDS operation is served 32 lanes per pass).
     constexpr bool SHARE4 = (MODE == kModeWSum || MODE == kModeTan);
+    constexpr bool SHARE2 = (MODE == kModeCount3D || MODE == kModeProj);
     const bool shared = (MODE == kModeXYZ) || (!F64 && A.shared_count);
     const uint32_t step_r = (MODE == kModeXYZ)
                                 ? (uint32_t)(Kp + 1) * 4u
-                                : (shared ? 4u : (SHARE4 ? 128u : (F64 ? 512u : 256u)));
-    const uint32_t lane_off =
-        shared ? 0u : (SHARE4 ? (uint32_t)(lane & 15) * 8u : (uint32_t)lane * (F64 ? 8u : 4u));
+                                : (shared ? 4u : ((SHARE4 || SHARE2) ? 128u : 512u));
+    const uint32_t lane_off = shared ? 0u
+                                     : (SHARE4 ? (uint32_t)(lane & 15) * 8u
+                                               : (SHARE2 ? (uint32_t)(lane & 31) * 4u : (uint32_t)lane * 8u));
     const uint32_t cnt_col = SHARE4 ? (uint32_t)(lane & 31) * 4u : 0u;  // SHARE4 count rows
     {
         const char* ws = static_cast<const char*>(L.ws);
@@ -686,7 +691,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     } else if (MODE == kModeCount3D || MODE == kModeProj || ALSO) {
         uint64_t* out = L.pcount + g * Kr;
         for (int s = 0; s < Kr; s++) {
-            const uint32_t c = SHARE4 ? (lane < 32 ? ch32[s * 32 + lane] : 0u) : ch32[s * 64 + lane];
+            const uint32_t c = (SHARE4 || SHARE2) ? (lane < 32 ? ch32[s * 32 + lane] : 0u) : ch32[s * 64 + lane];
             const uint64_t t = wave_sum_u64((uint64_t)c);
             if (lane == 0) out[s] = t;
         }
@@ -765,7 +770,7 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
         case kModeCount3D:
         case kModeProj:
             if (Kr > 64) { A.shared_count = 1; wave_bytes = (Kr + 1) * 4; }
-            else wave_bytes = (Kr + 1) * 64 * 4;
+            else wave_bytes = (Kr + 1) * 32 * 4;  // rows shared by lanes l and l+32
             break;
         case kModeXYZ: wave_bytes = (Kr + 1) * (Kp + 1) * 4; break;
         case kModeJac:
