@@ -166,40 +166,43 @@ __global__ void __launch_bounds__(256) k_radix_scan(uint32_t* __restrict__ tile_
     if (threadIdx.x == 255) digit_totals[blockIdx.x] = part[255];
 }
 
-// One wavefront per tile.  Keys are placed in tile order, and inside a tile
-// in index order: a lane's rank among the lanes holding the same digit comes
-// from eight ballots, so the sort is stable without any atomics.
-__global__ void __launch_bounds__(64) k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                      uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                      int64_t n, int shift, const uint32_t* __restrict__ tile_counts,
-                                                      const uint32_t* __restrict__ digit_totals, int ntiles) {
-    __shared__ uint32_t run[kRadixDigits];
-    const int lane = threadIdx.x;
+// One workgroup (4 wavefronts) per tile of kSortTile keys.  Each wave ranks
+// its 512 keys stably (a lane's rank among lanes with the same digit comes
+// from eight ballots; per-wave digit counters carry it across rounds), the
+// tile is staged in LDS in digit order, and the staged keys go out in runs
+// of equal digits, so consecutive threads write consecutive addresses.
+constexpr int kScatterWaves = 4;
+constexpr int kPerWave = kSortTile / kScatterWaves;  // 512 keys, 8 rounds of 64
+
+__global__ void __launch_bounds__(256) k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                       uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                       int64_t n, int shift, const uint32_t* __restrict__ tile_counts,
+                                                       const uint32_t* __restrict__ digit_totals, int ntiles) {
+    __shared__ uint32_t skey[kSortTile];
+    __shared__ uint32_t sval[kSortTile];
+    __shared__ uint32_t wcnt[kScatterWaves][kRadixDigits];  // per-wave digit counts, then offsets
+    __shared__ uint32_t toff[kRadixDigits];                 // first staged position of each digit
+    __shared__ uint32_t gbase[kRadixDigits];                // global position of the tile's first key per digit
+    __shared__ uint32_t scan[kRadixDigits];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = tid >> 6;
     const int tile = blockIdx.x;
-    // digit bases: exclusive scan of the 256 totals, 4 digits per lane
-    uint32_t t[4];
-    uint32_t local = 0;
-    for (int q = 0; q < 4; q++) { t[q] = digit_totals[lane * 4 + q]; local += t[q]; }
-    uint32_t incl = local;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t v = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += v;
-    }
-    uint32_t b = incl - local;
-    for (int q = 0; q < 4; q++) {
-        const int d = lane * 4 + q;
-        run[d] = b + tile_counts[(int64_t)d * ntiles + tile];
-        b += t[q];
-    }
-    __syncthreads();
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int64_t base = (int64_t)tile * kSortTile;
-    for (int r = 0; r < kSortTile / kWave; r++) {
-        const int64_t idx = base + r * kWave + lane;
-        if (base + r * kWave >= n) break;
-        const bool valid = idx < n;
-        const uint32_t key = valid ? kin[idx] : 0u;
-        const uint32_t val = valid ? (vin ? vin[idx] : (uint32_t)idx) : 0u;
+    const int tn = (int)min((int64_t)kSortTile, n - base);
+
+    for (int i = tid; i < kScatterWaves * kRadixDigits; i += 256) (&wcnt[0][0])[i] = 0u;
+    __syncthreads();
+
+    // ---- stable ranks within each wave
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t kk[kPerWave / 64], vv[kPerWave / 64], rk[kPerWave / 64];
+#pragma unroll
+    for (int r = 0; r < kPerWave / 64; r++) {
+        const int li = wv * kPerWave + r * 64 + lane;  // position in the tile
+        const bool valid = li < tn;
+        const uint32_t key = valid ? kin[base + li] : 0u;
+        const uint32_t val = valid ? (vin ? vin[base + li] : (uint32_t)(base + li)) : 0u;
         const uint32_t d = (key >> shift) & (kRadixDigits - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -208,15 +211,76 @@ __global__ void __launch_bounds__(64) k_radix_scatter(const uint32_t* __restrict
             const uint64_t bm = __ballot(on);
             peers &= on ? bm : ~bm;
         }
-        const uint32_t rank = (uint32_t)__popcll(peers & lt);
-        const uint32_t pos = run[d] + rank;
-        if (valid) {
-            kout[pos] = key;
-            vout[pos] = val;
+        const uint32_t before = (uint32_t)__popcll(peers & lt);
+        const uint32_t cnt = valid ? wcnt[wv][d] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && before == 0) wcnt[wv][d] = cnt + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        kk[r] = key;
+        vv[r] = val;
+        rk[r] = valid ? cnt + before : 0xffffffffu;
+    }
+    __syncthreads();
+
+    // ---- digit offsets in the staged tile: digits in order, waves in order
+    {
+        const int d = tid;  // 256 threads, one digit each
+        uint32_t tot = 0;
+        for (int w = 0; w < kScatterWaves; w++) tot += wcnt[w][d];
+        scan[d] = tot;
+        __syncthreads();
+        for (int off = 1; off < kRadixDigits; off <<= 1) {
+            const uint32_t v = d >= off ? scan[d - off] : 0u;
+            __syncthreads();
+            scan[d] += v;
+            __syncthreads();
         }
-        __builtin_amdgcn_wave_barrier();
-        if (valid && rank == 0) run[d] = pos + (uint32_t)__popcll(peers);
-        __builtin_amdgcn_wave_barrier();
+        const uint32_t start = scan[d] - tot;
+        toff[d] = start;
+        uint32_t run = start;
+        for (int w = 0; w < kScatterWaves; w++) {
+            const uint32_t c = wcnt[w][d];
+            wcnt[w][d] = run;
+            run += c;
+        }
+        // global base: all keys of smaller digits, then this digit's keys of earlier tiles
+        gbase[d] = tile_counts[(int64_t)d * ntiles + tile];
+    }
+    {
+        // exclusive scan of the 256 digit totals (same for every tile)
+        const int d = tid;
+        const uint32_t t = digit_totals[d];
+        __syncthreads();
+        scan[d] = t;
+        __syncthreads();
+        for (int off = 1; off < kRadixDigits; off <<= 1) {
+            const uint32_t v = d >= off ? scan[d - off] : 0u;
+            __syncthreads();
+            scan[d] += v;
+            __syncthreads();
+        }
+        gbase[d] += scan[d] - t;
+    }
+    __syncthreads();
+
+    // ---- stage in digit order
+#pragma unroll
+    for (int r = 0; r < kPerWave / 64; r++) {
+        if (rk[r] == 0xffffffffu) continue;
+        const uint32_t d = (kk[r] >> shift) & (kRadixDigits - 1);
+        const uint32_t pos = wcnt[wv][d] + rk[r];
+        skey[pos] = kk[r];
+        sval[pos] = vv[r];
+    }
+    __syncthreads();
+
+    // ---- write out: runs of equal digits land on consecutive addresses
+    for (int i = tid; i < tn; i += 256) {
+        const uint32_t key = skey[i];
+        const uint32_t d = (key >> shift) & (kRadixDigits - 1);
+        const uint32_t pos = gbase[d] + (uint32_t)i - toff[d];
+        kout[pos] = key;
+        vout[pos] = sval[i];
     }
 }
 
@@ -292,7 +356,7 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, c
                            sb.digit_totals);
         // the last pass writes the permutation straight into out.perm
         uint32_t* vdst = (pass == passes - 1) ? out.perm : sb.vals[cur ^ 1];
-        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(64), 0, s, sb.keys[cur],
+        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(256), 0, s, sb.keys[cur],
                            pass == 0 ? (const uint32_t*)nullptr : sb.vals[cur], sb.keys[cur ^ 1], vdst, n,
                            shift, sb.tile_counts, sb.digit_totals, ntiles);
         cur ^= 1;
