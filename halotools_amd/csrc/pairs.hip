@@ -306,6 +306,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     const uint32_t shr = L.br.shift, bshr = L.br.basesh, nbr1 = L.br.nb - 1;
     const uint32_t shp = L.bp.shift, bshp = L.bp.basesh, nbp1 = L.bp.nb - 1;
     const uint32_t trash_r = (uint32_t)Kr * step_r;  // projected: pairs beyond pi_max
+    const uint32_t trash_p = (uint32_t)Kp * 4u;      // xy_z: beyond the last pi edge
 
     uint64_t evals = 0;
     // Shared (per-wave) uint32 histograms can receive up to 64 increments per
@@ -345,10 +346,14 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
             const uint32_t a = bin_addr<MULTI>(dxy, s_tr, shr, bshr, nbr1, step_r, gedges_r);
             return (dz2 <= pimax2) ? a : trash_r;
         } else if (MODE == kModeXYZ) {
+            // pairs outside (rp_max, pi_max) are marked trash_r and skip the
+            // atomics: in one shared per-wave histogram they would all pile
+            // onto the same trash cells
             const double dxy = dx * dx + dy * dy;
             const double dz2 = dz * dz;
-            return bin_addr<MULTI>(dxy, s_tr, shr, bshr, nbr1, step_r, gedges_r) +
-                   bin_addr<MULTI>(dz2, s_tp, shp, bshp, nbp1, 4u, gedges_p);
+            const uint32_t ar = bin_addr<MULTI>(dxy, s_tr, shr, bshr, nbr1, step_r, gedges_r);
+            const uint32_t ap = bin_addr<MULTI>(dz2, s_tp, shp, bshp, nbp1, 4u, gedges_p);
+            return (ar < trash_r && ap < trash_p) ? ar + ap : trash_r;
         } else {
             const double dsq = dx * dx + dy * dy + dz * dz;
             return bin_addr<MULTI>(dsq, s_tr, shr, bshr, nbr1, step_r, gedges_r);
@@ -503,7 +508,8 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
             }
         } else {
 #pragma unroll
-            for (int u = 0; u < B; u++) accumulate(a[u], off[u] >> 3, rinc, false);
+            for (int u = 0; u < B; u++)
+                if (MODE != kModeXYZ || a[u] != trash_r) accumulate(a[u], off[u] >> 3, rinc, false);
         }
     };
     auto drain = [&]() __attribute__((always_inline)) {
@@ -675,7 +681,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                         __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(wsm + A.cnt_off + a[u] + cnt_col), inc,
                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
-                } else if (MODE != kModeTan) {
+                } else if (MODE != kModeTan && (MODE != kModeXYZ || a[u] != trash_r)) {
                     accumulate(a[u], 0u, inc, false);
                 }
             }
