@@ -17,8 +17,6 @@
 #include "hp_internal.h"
 
 namespace hp {
-int launch_reduce_counts_s(const uint64_t* partial, int nr, int np, const MockGeom* geom, int nmocks, int side,
-                           int64_t max_groups, uint64_t* scratch, int64_t* counts, hipStream_t s);
 int launch_reduce_sums_s(const double* partial, int nr, const MockGeom* geom, int nmocks, int side, int64_t max_groups,
                          double* scratch, double* sums, hipStream_t s);
 int launch_reduce_evals_s(const uint64_t* pevals, const MockGeom* geom, int nmocks, int side, int64_t max_groups,
@@ -148,6 +146,7 @@ struct Plan {
     bool elim = false;
     size_t o_corr = 0, o_rows_tmp = 0;
     bool sym = false;  // auto-count (unweighted or tangent): symmetric evaluation
+    int crep = 1;
     bool tangent = false;
     size_t o_ptan = 0;
     size_t total = 0;
@@ -158,6 +157,7 @@ static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 constexpr double kCellTarget = 4096.0;  // most stream points per default fine cell
 constexpr int kHilbertMaxBits = 8;   // lane-ordering grid <= 256^3 cells per mock
+constexpr int kCountReplicas = 256;  // count accumulator copies per mock
 
 static int bits_for(uint64_t range) {
     int b = 1;
@@ -358,12 +358,17 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     P.o_tiles = take((size_t)ntiles * kRadixDigits * 4);
     P.o_digits = take((size_t)kRadixDigits * 4);
     const int64_t gA = std::max<int64_t>(P.ngroups[0], 1);
-    if (P.counts) P.o_pcount = take((size_t)gA * (size_t)P.nb_count * 8);
+    // count accumulators: up to kCountReplicas copies per mock (a group adds
+    // into copy group % crep), so the atomics of many groups spread over cache
+    // lines; wide histograms (xy_z) spread over their own bins already
+    P.crep = (int)std::max<int64_t>(
+        1, std::min<int64_t>({(int64_t)kCountReplicas, P.max_groups[0], (int64_t)(2048 / std::max(P.nb_count, 1))}));
+    if (P.counts) P.o_pcount = take((size_t)P.M * (size_t)P.crep * (size_t)P.nb_count * 8);
     if (P.sums) P.o_psum = take((size_t)gA * (size_t)P.Kr * 8);
     if (P.tangent) P.o_ptan = take((size_t)gA * (size_t)P.Kr * 8);
     P.o_pevals = take((size_t)gA * 8);
     const int64_t red = reduce_scratch_elems(std::max<int64_t>(P.max_groups[0], 1), P.M,
-                                             std::max(P.nb_count, P.Kr));
+                                             P.Kr);  // sums (Kr) and evals (1)
     P.o_red = take((size_t)red * 8);
     // auto-count with shared weights: rows2 = rows1 + seam corrections, no swapped pass
     P.elim = P.rows2 && P.same && P.marked && r->wfunc == 1;
@@ -510,6 +515,9 @@ static int run(const hp_request* r, int kind) {
         L.corr = corr;
         launch_fill_u32(reinterpret_cast<uint32_t*>(corr), 0u, P.n[0] * (int64_t)P.Kr * 2, s);
     }
+    L.crep = P.crep;
+    if (P.counts)
+        launch_fill_u32(reinterpret_cast<uint32_t*>(L.pcount), 0u, (int64_t)P.M * P.crep * P.nb_count * 2, s);
     int rc = 0;
     if (r->pair_events[0]) hipEventRecord(static_cast<hipEvent_t>(r->pair_events[0]), s);
     if (!(P.marked && !P.sums && !P.counts && !P.rows1 && !P.elim)) {
@@ -543,8 +551,7 @@ static int run(const hp_request* r, int kind) {
     if (r->pair_events[1]) hipEventRecord(static_cast<hipEvent_t>(r->pair_events[1]), s);
     uint64_t* red = reinterpret_cast<uint64_t*>(ws + P.o_red);
     if (P.counts) {
-        if (launch_reduce_counts_s(L.pcount, P.Kr, kind == HP_NPAIRS_XY_Z ? P.Kp : 0, dgeom, P.M, 0,
-                                   P.max_groups[0], red, r->counts, s))
+        if (launch_prefix_counts(L.pcount, P.Kr, kind == HP_NPAIRS_XY_Z ? P.Kp : 0, P.M, P.crep, r->counts, s))
             return fail("reduction launch failed");
     }
     if (P.sums) {

@@ -109,7 +109,8 @@ struct PairLaunch {
     BinSpec br;            // radial / rp bins
     BinSpec bp;            // pi bins (xy_z) or pi_max (projected, nbins = 1)
     const void* ws;        // workspace base (edges / tables)
-    uint64_t* pcount;      // [ngroups][nbins_count]
+    uint64_t* pcount;      // [nmocks][crep][nbins_count], zeroed, accumulated with atomics
+    int crep;              // count replicas per mock (spread the atomics over cache lines)
     double* psum;          // [ngroups][nr]
     double* ptan;          // tangent mode: [ngroups][nr]
     double* rows;          // [n_lane_total][nr] (original order) or nullptr
@@ -123,6 +124,7 @@ enum Mode : int { kModeCount3D = 0, kModeProj = 1, kModeXYZ = 2, kModeWSum = 3, 
 int launch_pairs(const PairLaunch& L, hipStream_t s);
 
 // reduce.hip
+int launch_prefix_counts(const uint64_t* acc, int nr, int np, int nmocks, int crep, int64_t* counts, hipStream_t s);
 int64_t segment_sum_scratch(const int64_t* offsets, int nmocks);
 int launch_segment_sum(const double* v, const int64_t* offsets, int nmocks, double* out, double* scratch,
                        hipStream_t s);

@@ -313,10 +313,12 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     // stream point; flush them into the group's uint64 row before they could wrap.
     const int shared_bins = (MODE == kModeXYZ) ? Kr * Kp : Kr;
     const int row_len = (MODE == kModeXYZ) ? Kp + 1 : 1;
-    uint64_t* const shared_out = L.pcount + g * (int64_t)shared_bins;
+    // per-mock count accumulator: integer adds are exact in any order, so the
+    // groups add their histograms with atomics (no per-group partials)
+    uint64_t* const cacc =
+        L.pcount ? L.pcount + ((int64_t)m * L.crep + (int64_t)(gl % L.crep)) * shared_bins : nullptr;
     uint32_t* const ch32 = reinterpret_cast<uint32_t*>(wsm + A.cnt_off);
     uint64_t bound = 0;
-    bool flushed = false;
 
     // Pairs are processed in batches: the stream points of a batch arrive with
     // one scalar-load wait, all distances are formed, then all bucket-table
@@ -551,11 +553,12 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                     if (bound >= 0xF0000000ull) {
                         for (int s = lane; s < shared_bins; s += 64) {
                             const int src = (MODE == kModeXYZ) ? (s / Kp) * row_len + s % Kp : s;
-                            shared_out[s] = (flushed ? shared_out[s] : 0ull) + ch32[src];
+                            if (ch32[src])
+                                __hip_atomic_fetch_add(cacc + s, (uint64_t)ch32[src], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
                         }
                         for (int s = lane; s < (Kr + 1) * row_len; s += 64) ch32[s] = 0u;
-                        flushed = true;
-                        bound = (uint64_t)cnt * (uint64_t)(je - jb);
+                        bound = (uint64_t)cnt * (uint64_t)(je - jb) * (sym ? 2u : 1u);
                     }
                 }
                 // x2shift of the reference: the shift that brings the sample-2
@@ -692,14 +695,14 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     if (shared) {
         for (int s = lane; s < shared_bins; s += 64) {
             const int src = (MODE == kModeXYZ) ? (s / Kp) * row_len + s % Kp : s;
-            shared_out[s] = (flushed ? shared_out[s] : 0ull) + ch32[src];
+            if (ch32[src])
+                __hip_atomic_fetch_add(cacc + s, (uint64_t)ch32[src], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     } else if (MODE == kModeCount3D || MODE == kModeProj || ALSO) {
-        uint64_t* out = L.pcount + g * Kr;
         for (int s = 0; s < Kr; s++) {
             const uint32_t c = (SHARE4 || SHARE2) ? (lane < 32 ? ch32[s * 32 + lane] : 0u) : ch32[s * 64 + lane];
             const uint64_t t = wave_sum_u64((uint64_t)c);
-            if (lane == 0) out[s] = t;
+            if (lane == s % 64 && t) __hip_atomic_fetch_add(cacc + s, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     const double* fh = reinterpret_cast<const double*>(wsm);

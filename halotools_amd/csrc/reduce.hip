@@ -80,6 +80,45 @@ __global__ void __launch_bounds__(256) k_reduce_s2(const T* __restrict__ s1, int
     }
 }
 
+// Counts: the pair kernel accumulates each mock's histogram directly (exact
+// integer atomics); this turns it into the reference's cumulative counts.
+__global__ void __launch_bounds__(256) k_prefix_counts(const uint64_t* __restrict__ acc, int nr, int np, int crep,
+                                                       int64_t* __restrict__ out) {
+    const int m = blockIdx.x;
+    const int nb = np > 0 ? nr * np : nr;
+    const uint64_t* a = acc + (int64_t)m * crep * nb;
+    int64_t* o = out + (int64_t)m * nb;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        uint64_t t = 0;
+        for (int r = 0; r < crep; r++) t += a[(int64_t)r * nb + b];
+        o[b] = (int64_t)t;
+    }
+    __threadfence_block();
+    __syncthreads();
+    if (np <= 0) {
+        if (threadIdx.x == 0) {
+            int64_t run = 0;
+            for (int k = 0; k < nr; k++) { run += o[k]; o[k] = run; }
+        }
+    } else {
+        for (int k = threadIdx.x; k < nr; k += blockDim.x) {
+            int64_t run = 0;
+            for (int q = 0; q < np; q++) { run += o[k * np + q]; o[k * np + q] = run; }
+        }
+        __threadfence_block();
+        __syncthreads();
+        for (int q = threadIdx.x; q < np; q += blockDim.x) {
+            int64_t run = 0;
+            for (int k = 0; k < nr; k++) { run += o[k * np + q]; o[k * np + q] = run; }
+        }
+    }
+}
+
+int launch_prefix_counts(const uint64_t* acc, int nr, int np, int nmocks, int crep, int64_t* counts, hipStream_t s) {
+    hipLaunchKernelGGL(k_prefix_counts, dim3(nmocks), dim3(256), 0, s, acc, nr, np, crep, counts);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // Plain per-mock totals (no prefix): used for the candidate-evaluation counter.
 __global__ void __launch_bounds__(256) k_reduce_plain(const uint64_t* __restrict__ s1, const MockGeom* __restrict__ geom,
                                                       int side, int nchunk_max, int64_t* __restrict__ out) {
@@ -98,16 +137,6 @@ static int64_t nchunk_of(int64_t max_groups) { return (max_groups + kChunk - 1) 
 int64_t reduce_scratch_elems(int64_t max_groups_per_mock, int nmocks, int nb) {
     const int64_t nc = nchunk_of(max_groups_per_mock) > 0 ? nchunk_of(max_groups_per_mock) : 1;
     return nc * nmocks * (int64_t)nb;
-}
-
-int launch_reduce_counts_s(const uint64_t* partial, int nr, int np, const MockGeom* geom, int nmocks, int side,
-                           int64_t max_groups, uint64_t* scratch, int64_t* counts, hipStream_t s) {
-    const int nb = np > 0 ? nr * np : nr;
-    const int nc = (int)(nchunk_of(max_groups) > 0 ? nchunk_of(max_groups) : 1);
-    hipLaunchKernelGGL(k_reduce_s1<uint64_t>, dim3(nc, nmocks), dim3(256), 0, s, partial, nb, geom, side, nc, scratch);
-    hipLaunchKernelGGL((k_reduce_s2<uint64_t, int64_t>), dim3(nmocks), dim3(256), 0, s, scratch, nr, np, geom, side, nc,
-                       counts);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_reduce_sums_s(const double* partial, int nr, const MockGeom* geom, int nmocks, int side, int64_t max_groups,
