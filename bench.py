@@ -195,6 +195,28 @@ def bench_other_configs(dev, steps):
                      "kernel_evals_per_s": evals / ks, "fp64_peak_frac": 8.0 * evals / ks / 1e12 / FP64_PEAK_TFLOPS}
     del t3, cols
 
+    # per-object counts on the headline's clustered box: (N, 20) int64 rows
+    from halotools_amd.mock_observables import npairs_per_object_3d
+    p2 = mocks.nfw_clustered_box(200000, 1000.0, np.random.default_rng([43, 2]))
+    t2 = torch.from_numpy(np.ascontiguousarray(p2)).to(dev)
+    rb2 = mocks.config_rbins(2)
+    rows, s_obj = timed(lambda: npairs_per_object_3d(t2, t2, rb2, period=1000.0))
+    cols2 = tuple(t2[:, i].contiguous() for i in range(3))
+    ev = _events(torch)
+    m2 = dict(n1=len(p2), n2=len(p2), period=[1000.0] * 3, pbc=True)
+    res = D.run(E.HP_NPAIRS_PER_OBJECT_3D, [m2], cols2, cols2, rb2 * rb2, same=True, want_evals=True, device=dev,
+                pair_events=(ev[0].cuda_event, ev[1].cuda_event))
+    torch.cuda.synchronize()
+    assert torch.equal(res.counts, rows)
+    ks = ev[0].elapsed_time(ev[1]) * 1e-3
+    evals = int(res.evals[0].item())
+    out["npairs_per_object_3d"] = {
+        "workload": "%d-point NFW-clustered box (as the headline), L=1000, 20 log edges 0.1-50, rows (N, 20)"
+                    % len(p2),
+        "ms_per_call": s_obj * 1e3, "pair_kernel_ms": ks * 1e3, "kernel_evals": evals,
+        "kernel_evals_per_s": evals / ks, "fp64_peak_frac": 8.0 * evals / ks / 1e12 / FP64_PEAK_TFLOPS}
+    del t2, cols2, rows, res
+
     rng = np.random.default_rng([43, 4])
     p4 = mocks.uniform_box(1000000, 1000.0, rng)
     logm = torch.tensor(rng.uniform(11, 15, len(p4)), device=dev)

@@ -140,13 +140,15 @@ def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0
     req.p2 = p2.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if p2 is not None else None
 
     res = Result()
+    obj = kind == E.HP_NPAIRS_PER_OBJECT_3D
     ncount = nr * (len(p2) if kind == E.HP_NPAIRS_XY_Z else 1)
     need_counts = (not marked) or also_counts
     # counts and the status word share one buffer so one copy brings both back
-    cbuf = torch.empty(M * ncount + 1 if need_counts else 1, dtype=torch.int64, device=device)
+    clen = n1_tot * nr if obj else M * ncount
+    cbuf = torch.empty(clen + 1 if need_counts else 1, dtype=torch.int64, device=device)
     res.status = cbuf[-1:]
     if need_counts:
-        shape = (M, nr, len(p2)) if kind == E.HP_NPAIRS_XY_Z else (M, nr)
+        shape = (M, nr, len(p2)) if kind == E.HP_NPAIRS_XY_Z else ((n1_tot, nr) if obj else (M, nr))
         res.counts = cbuf[:-1].view(*shape)
         req.counts = cbuf.data_ptr()
     else:
@@ -176,7 +178,8 @@ def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0
         req.build_events[0], req.build_events[1] = build_events
     req.tile_waves = int(tile_waves)
     res.fn = {E.HP_NPAIRS_3D: lib.hp_npairs_3d, E.HP_NPAIRS_PROJECTED: lib.hp_npairs_projected,
-              E.HP_NPAIRS_XY_Z: lib.hp_npairs_xy_z, E.HP_MARKED_NPAIRS_3D: lib.hp_marked_npairs_3d}[kind]
+              E.HP_NPAIRS_XY_Z: lib.hp_npairs_xy_z, E.HP_MARKED_NPAIRS_3D: lib.hp_marked_npairs_3d,
+              E.HP_NPAIRS_PER_OBJECT_3D: lib.hp_npairs_per_object_3d}[kind]
     res.req = req
     res.device = device
     # keep every host and device buffer the request points to alive with it

@@ -148,6 +148,7 @@ struct Plan {
     bool sym = false;  // auto-count (unweighted or tangent): symmetric evaluation
     int crep = 1;
     bool tangent = false;
+    bool obj = false;  // per-object counts
     size_t o_ptan = 0;
     size_t total = 0;
     std::vector<unsigned char> consts;  // bytes [0, o_gm[0])
@@ -174,10 +175,11 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     if (!r->mocks) return fail("mocks must not be null");
     P.same = (r->flags & HP_SAME_SAMPLES) != 0;
     P.marked = (kind == HP_MARKED_NPAIRS_3D);
+    P.obj = (kind == HP_NPAIRS_PER_OBJECT_3D);
     P.rows1 = P.marked && (r->flags & HP_ROWS1);
     P.rows2 = P.marked && (r->flags & HP_ROWS2);
     P.sums = P.marked && !(r->flags & HP_NO_SUMS);
-    P.counts = !P.marked || (r->flags & HP_ALSO_COUNTS);
+    P.counts = (!P.marked && !P.obj) || (P.marked && (r->flags & HP_ALSO_COUNTS));
     P.tangent = P.marked && (r->flags & HP_TANGENT);
     P.nw = P.marked ? r->nw : 0;
     if (r->tile_waves != 0 && r->tile_waves != 1 && r->tile_waves != 2 && r->tile_waves != 4 && r->tile_waves != 8)
@@ -378,7 +380,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     }
     // unweighted auto-counts evaluate each unshifted unordered pair once; the
     // Hilbert ranks reuse the sort scratch (free once both sides are built)
-    P.sym = P.same && (!P.marked || P.tangent);
+    P.sym = P.same && !P.obj && (!P.marked || P.tangent);
     P.total = align_up(off, 256);
 
     // ---- constant blob (geometry, edges, tables) that goes to the workspace
@@ -427,7 +429,7 @@ static int run(const hp_request* r, int kind) {
         return fail("sample2 coordinates must not be null");
     if (P.marked && ((P.n[0] > 0 && !r->w1) || (!P.same && P.n[1] > 0 && !r->w2)))
         return fail("weights must not be null");
-    if (P.counts && !r->counts) return fail("counts output must not be null");
+    if ((P.counts || P.obj) && !r->counts) return fail("counts output must not be null");
     if (P.sums && !r->sums) return fail("sums output must not be null");
     if (P.tangent && !r->tangents) return fail("tangents output must not be null");
     if (P.rows1 && !r->rows1) return fail("rows1 output must not be null");
@@ -501,6 +503,10 @@ static int run(const hp_request* r, int kind) {
         case HP_NPAIRS_3D: L.mode = kModeCount3D; break;
         case HP_NPAIRS_PROJECTED: L.mode = kModeProj; break;
         case HP_NPAIRS_XY_Z: L.mode = kModeXYZ; break;
+        case HP_NPAIRS_PER_OBJECT_3D:
+            L.mode = kModeObj;
+            L.orows = r->counts;
+            break;
         default:
             L.mode = P.tangent ? kModeTan
                                : ((r->wfunc == 1 && P.nw == 1 && (P.rows1 || P.rows2)) ? kModeJac : kModeWSum);
@@ -590,6 +596,7 @@ int hp_npairs_3d(const hp_request* req) { return run(req, HP_NPAIRS_3D); }
 int hp_npairs_projected(const hp_request* req) { return run(req, HP_NPAIRS_PROJECTED); }
 int hp_npairs_xy_z(const hp_request* req) { return run(req, HP_NPAIRS_XY_Z); }
 int hp_marked_npairs_3d(const hp_request* req) { return run(req, HP_MARKED_NPAIRS_3D); }
+int hp_npairs_per_object_3d(const hp_request* req) { return run(req, HP_NPAIRS_PER_OBJECT_3D); }
 
 int hp_marked_backward(const double* rows, int32_t nr, const double* gsuffix, const int64_t* offsets,
                        int32_t nmocks, double* grad, void* stream) {

@@ -113,6 +113,7 @@ struct PairLaunch {
     int crep;              // count replicas per mock (spread the atomics over cache lines)
     double* psum;          // [ngroups][nr]
     double* ptan;          // tangent mode: [ngroups][nr]
+    int64_t* orows;        // per-object mode: [n_lane_total][nr] cumulative counts (input order)
     double* rows;          // [n_lane_total][nr] (original order) or nullptr
     double* corr;          // auto-counts: rows2 - rows1 corrections from seam pairs, or nullptr
     uint64_t* pevals;      // [ngroups]
@@ -120,7 +121,8 @@ struct PairLaunch {
     // unshifted unordered pair is then evaluated once and counted twice
     const uint32_t* hrank;
 };
-enum Mode : int { kModeCount3D = 0, kModeProj = 1, kModeXYZ = 2, kModeWSum = 3, kModeJac = 4, kModeTan = 5 };
+enum Mode : int { kModeCount3D = 0, kModeProj = 1, kModeXYZ = 2, kModeWSum = 3, kModeJac = 4, kModeTan = 5,
+                  kModeObj = 6 };
 int launch_pairs(const PairLaunch& L, hipStream_t s);
 
 // reduce.hip

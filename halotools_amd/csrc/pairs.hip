@@ -177,6 +177,7 @@ static_assert(kRing >= kBatch - 1 + 64, "survivor ring too small for the batch s
 template <int MODE>
 constexpr int kWavesPerSimd = (MODE == kModeJac) ? 1 : ((MODE == kModeWSum || MODE == kModeTan) ? 4
                                                                                  : (MODE == kModeXYZ ? 5 : 6));
+// per-object counts: one u32 row set per lane (each lane owns its point's counts)
 
 template <int MODE, int SWAP, int ALSO, int MULTI, int WAVES>
 __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const PairArgs A) {
@@ -205,10 +206,11 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     const bool shared = (MODE == kModeXYZ) || (!F64 && A.shared_count);
     const uint32_t step_r = (MODE == kModeXYZ)
                                 ? (uint32_t)(Kp + 1) * 4u
-                                : (shared ? 4u : ((SHARE4 || SHARE2) ? 128u : 512u));
-    const uint32_t lane_off = shared ? 0u
-                                     : (SHARE4 ? (uint32_t)(lane & 15) * 8u
-                                               : (SHARE2 ? (uint32_t)(lane & 31) * 4u : (uint32_t)lane * 8u));
+                                : (shared ? 4u : ((SHARE4 || SHARE2) ? 128u : (MODE == kModeObj ? 256u : 512u)));
+    const uint32_t lane_off =
+        shared ? 0u
+               : (SHARE4 ? (uint32_t)(lane & 15) * 8u
+                         : (SHARE2 ? (uint32_t)(lane & 31) * 4u : (uint32_t)lane * (MODE == kModeObj ? 4u : 8u)));
     const uint32_t cnt_col = SHARE4 ? (uint32_t)(lane & 31) * 4u : 0u;  // SHARE4 count rows
     {
         const char* ws = static_cast<const char*>(L.ws);
@@ -714,6 +716,15 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
             }
         }
     }
+    if (MODE == kModeObj && act) {
+        // the lane's own counts, cumulative over bins, at its input position
+        int64_t* row = L.orows + q * Kr;
+        int64_t run = 0;
+        for (int s = 0; s < Kr; s++) {
+            run += (int64_t)ch32[s * 64 + lane];
+            row[s] = run;
+        }
+    }
     if (MODE == kModeTan) {
         // rows of w_i w_j and of t_i w_j + w_i t_j, doubled under symmetry
         const double* th = reinterpret_cast<const double*>(wsm + A.tan_off);
@@ -782,6 +793,7 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
             else wave_bytes = (Kr + 1) * 32 * 4;  // rows shared by lanes l and l+32
             break;
         case kModeXYZ: wave_bytes = (Kr + 1) * (Kp + 1) * 4; break;
+        case kModeObj: wave_bytes = (Kr + 1) * 64 * 4; break;
         case kModeJac:
             A.cnt_off = (Kr + 1) * 64 * 8;
             wave_bytes = A.cnt_off + (L.also_count ? (Kr + 1) * 64 * 4 : 0);
@@ -838,6 +850,7 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
         case kModeCount3D: return launch_m<kModeCount3D, 0, 0>(A, waves, multi, s);
         case kModeProj: return launch_m<kModeProj, 0, 0>(A, waves, multi, s);
         case kModeXYZ: return launch_m<kModeXYZ, 0, 0>(A, waves, multi, s);
+        case kModeObj: return launch_m<kModeObj, 0, 0>(A, waves, multi, s);
         case kModeWSum:
             return also ? launch_m<kModeWSum, 0, 1>(A, waves, multi, s) : launch_m<kModeWSum, 0, 0>(A, waves, multi, s);
         case kModeJac:

@@ -48,7 +48,8 @@ enum hp_kind {
     HP_NPAIRS_3D = 0,        /* PC/cpairs/npairs_3d_engine.pyx:17            */
     HP_NPAIRS_PROJECTED = 1, /* PC/cpairs/npairs_projected_engine.pyx:17     */
     HP_NPAIRS_XY_Z = 2,      /* PC/cpairs/npairs_xy_z_engine.pyx:17          */
-    HP_MARKED_NPAIRS_3D = 3  /* PC/marked_cpairs/marked_npairs_3d_engine.pyx:22 */
+    HP_MARKED_NPAIRS_3D = 3, /* PC/marked_cpairs/marked_npairs_3d_engine.pyx:22 */
+    HP_NPAIRS_PER_OBJECT_3D = 4 /* PC/cpairs/npairs_per_object_3d_engine.pyx:19 */
 };
 
 /* Bit flags of hp_request.flags */
@@ -132,6 +133,11 @@ int hp_npairs_projected(const hp_request* req);
 /* npairs_xy_z: counts[m][k][g] of pairs with dxy^2 <= r2[k], dz^2 <= p2[g].
  * Replaces PC/cpairs/npairs_xy_z_engine.pyx:17-196. */
 int hp_npairs_xy_z(const hp_request* req);
+
+/* npairs_per_object_3d: counts[i][k] (int64, [n1_total][nr], input order of
+ * sample 1) = number of sample-2 points j of i's mock with dsq_ij <= r2[k].
+ * Replaces PC/cpairs/npairs_per_object_3d_engine.pyx:19-208 and its unsorting. */
+int hp_npairs_per_object_3d(const hp_request* req);
 
 /* marked_npairs_3d: sums[m][k] of wfunc(w1_i, w2_j) over pairs within r2[k].
  * Replaces PC/marked_cpairs/marked_npairs_3d_engine.pyx:22-218 together with

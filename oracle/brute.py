@@ -105,3 +105,20 @@ def wnpairs_generic(sample1, sample2, r, weights1, weights2, wfunc, period=None)
                 if d[j] <= r[k]:
                     terms[k].append(w)
     return np.array([math.fsum(t) for t in terms])
+
+
+def npairs_per_object(sample1, sample2, rbins, period=None):
+    """Per-point counts, the reference's brute-force test helper
+    (PC/test_pair_counters/pure_python_npairs_per_object_3d.py:11-47) restated
+    with the minimum image and square roots: counts[i, k] = number of sample-2
+    points within rbins[k] of sample-1 point i."""
+    sample1 = np.atleast_2d(sample1)
+    sample2 = np.atleast_2d(sample2)
+    rbins = np.atleast_1d(rbins).astype("float64")
+    period = _period(period, 3)
+    out = np.zeros((len(sample1), len(rbins)), dtype=np.int64)
+    for i in range(len(sample1)):
+        m = _min_image(sample1[i], sample2, period)
+        d = np.sqrt(np.sum(m * m, axis=1))
+        out[i] = np.sum(d[:, None] <= rbins[None, :], axis=0)
+    return out

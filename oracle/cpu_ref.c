@@ -127,7 +127,7 @@ double ref_wfunc(int id, const double* w1, const double* w2) {
 /* The visitor is invoked for every (cell1, cell2) pair with the cell-2     */
 /* shift values, exactly where the Cython engines enter their i/j loops.    */
 
-enum { KIND_3D = 0, KIND_PROJ = 1, KIND_XYZ = 2, KIND_MARKED = 3 };
+enum { KIND_3D = 0, KIND_PROJ = 1, KIND_XYZ = 2, KIND_MARKED = 3, KIND_OBJ = 4 };
 
 typedef struct ref_job {
     int kind;
@@ -202,6 +202,16 @@ static void run_cells(const ref_job* J, int64_t first, int64_t last) {
                                     k = k - 1;
                                     if (k < 0) break;
                                 }
+                            } else if (J->kind == KIND_OBJ) {
+                                /* PC/cpairs/npairs_per_object_3d_engine.pyx:183-199: the
+                                 * counts of sorted sample-1 point i, row ifirst1 + i */
+                                const double dsq = dx * dx + dy * dy + dz * dz;
+                                int k = K - 1;
+                                while (bin_pass(dsq, J->r2[k])) {
+                                    J->counts[i * K + k] += 1;
+                                    k = k - 1;
+                                    if (k < 0) break;
+                                }
                             } else if (J->kind == KIND_MARKED) {
                                 /* PC/marked_cpairs/marked_npairs_3d_engine.pyx:207-216 */
                                 const double dsq = dx * dx + dy * dy + dz * dz;
@@ -267,6 +277,9 @@ static int run_threads(const ref_job* J, int nthreads) {
     const int64_t first = m->first_cell1, last = m->last_cell1;
     const int64_t ncells = last - first;
     const int64_t outlen = (J->kind == KIND_XYZ) ? (int64_t)J->K * J->G : J->K;
+    /* per-object rows of different cell-1 chunks are disjoint: workers share
+     * the output (the reference sums per-worker arrays that are zero elsewhere) */
+    const int shared_out = (J->kind == KIND_OBJ);
     if (nthreads <= 1 || ncells <= 1) {
         run_cells(J, first, last);
         return 0;
@@ -283,7 +296,7 @@ static int run_threads(const ref_job* J, int nthreads) {
         args[t].job = *J;
         if (J->kind == KIND_MARKED) {
             args[t].job.wcounts = (double*)calloc((size_t)outlen, sizeof(double));
-        } else {
+        } else if (!shared_out) {
             args[t].job.counts = (int64_t*)calloc((size_t)outlen, sizeof(int64_t));
         }
         args[t].first = start;
@@ -293,6 +306,7 @@ static int run_threads(const ref_job* J, int nthreads) {
     }
     for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
     for (int t = 0; t < nthreads; t++) {
+        if (shared_out) continue;
         for (int64_t k = 0; k < outlen; k++) {
             if (J->kind == KIND_MARKED) J->wcounts[k] += args[t].job.wcounts[k];
             else J->counts[k] += args[t].job.counts[k];
@@ -361,6 +375,22 @@ int ref_npairs_xy_z(const ref_mesh* m, const double* x1, const double* y1, const
     int rc = run_threads(&J, nthreads);
     free(r2);
     free(p2);
+    return rc;
+}
+
+/* counts: (n1, K) rows in the cell-sorted order of sample 1 */
+int ref_npairs_per_object_3d(const ref_mesh* m, const double* x1, const double* y1, const double* z1,
+                             const double* x2, const double* y2, const double* z2,
+                             const double* rbins, int K, int64_t n1, int nthreads, int64_t* counts) {
+    ref_job J;
+    memset(&J, 0, sizeof(J));
+    J.kind = KIND_OBJ; J.m = m;
+    J.x1 = x1; J.y1 = y1; J.z1 = z1; J.x2 = x2; J.y2 = y2; J.z2 = z2;
+    double* r2 = square_bins(rbins, K);
+    J.r2 = r2; J.K = K; J.counts = counts;
+    memset(counts, 0, sizeof(int64_t) * (size_t)K * (size_t)n1);
+    int rc = run_threads(&J, nthreads);
+    free(r2);
     return rc;
 }
 

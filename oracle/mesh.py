@@ -74,6 +74,7 @@ def load(fast=False):
     lib.ref_npairs_projected.argtypes = [ctypes.POINTER(_RefMeshC), P, P, P, P, P, P, P, I, D, I, P]
     lib.ref_npairs_xy_z.argtypes = [ctypes.POINTER(_RefMeshC), P, P, P, P, P, P, P, I, P, I, I, P]
     lib.ref_marked_npairs_3d.argtypes = [ctypes.POINTER(_RefMeshC), P, P, P, P, P, P, P, P, I, I, P, I, I, P]
+    lib.ref_npairs_per_object_3d.argtypes = [ctypes.POINTER(_RefMeshC), P, P, P, P, P, P, P, I, ctypes.c_int64, I, P]
     lib.ref_candidate_pairs.argtypes = [ctypes.POINTER(_RefMeshC)]
     lib.ref_candidate_pairs.restype = ctypes.c_int64
     lib.ref_set_mutation.argtypes = [I]
@@ -283,6 +284,26 @@ def npairs_3d(sample1, sample2, rbins, period=None, num_threads=1,
     rc = load(fast).ref_npairs_3d(ctypes.byref(mc), *_ptrs(cols), rbins.ctypes.data,
                                   len(rbins), int(num_threads), counts.ctypes.data)
     assert rc == 0
+    return counts
+
+
+def npairs_per_object_3d(sample1, sample2, rbins, period=None, num_threads=1,
+                         approx_cell1_size=None, approx_cell2_size=None, fast=False):
+    """Oracle for PC/npairs_per_object_3d.py:20-137: (Npts1, K) counts per
+    sample-1 point, unsorted back to input order like the engine
+    (PC/cpairs/npairs_per_object_3d_engine.pyx:202-205)."""
+    rbins = np.atleast_1d(rbins).astype("f8")
+    rmax = np.max(rbins)
+    mesh, cols = _prepare(sample1, sample2, (rmax, rmax, rmax), period,
+                          approx_cell1_size, approx_cell2_size, (rmax, rmax, rmax))
+    n1 = len(cols[0])
+    sorted_counts = np.zeros((n1, len(rbins)), dtype=np.int64)
+    mc = mesh.as_c()
+    rc = load(fast).ref_npairs_per_object_3d(ctypes.byref(mc), *_ptrs(cols), rbins.ctypes.data, len(rbins),
+                                             n1, int(num_threads), sorted_counts.ctypes.data)
+    assert rc == 0
+    counts = np.empty_like(sorted_counts)
+    counts[mesh.mesh1.idx_sorted] = sorted_counts
     return counts
 
 

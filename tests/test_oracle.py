@@ -64,6 +64,16 @@ def known_answer_failures():
     res = ORC.marked_npairs_3d(pts, pts, rbm, 1.0, np.ones(1000) * 3, np.ones(1000) * 3, 1)
     if not np.all(res == 9.0 * plain):
         fails.append("marked_uniform")
+    # per-object counts (PC/test_pair_counters/test_npairs_per_object_3d.py:19-57)
+    g5 = generate_3d_regular_mesh(5)
+    loc = generate_locus_of_3d_points(100, xc=0.101, yc=0.101, zc=0.101, seed=fixed_seed)
+    po = ORC.npairs_per_object_3d(g5, loc, [0.0001, 0.02], period=1)
+    if not (po.shape == (125, 2) and np.all(po[0] == (0, 100)) and np.all(po[1:] == 0)):
+        fails.append("per_object_locus0")
+    loc55 = generate_locus_of_3d_points(100, xc=g5[55, 0], yc=g5[55, 1], zc=g5[55, 2], seed=fixed_seed)
+    po = ORC.npairs_per_object_3d(g5, loc55, [0.0001, 0.02], period=1)
+    if not (np.all(po[55] == (0, 100)) and po.sum() == 100):
+        fails.append("per_object_locus55")
     return fails
 
 
@@ -84,6 +94,12 @@ def brute_force_failures():
         if not np.array_equal(ORC.npairs_projected(s1, s2, rp[1:], 0.1, period=per),
                               brute.xy_z_npairs(s1, s2, rp[1:], [0, 0.1], period=per)[:, 1]):
             fails.append("proj_%s" % per)
+    # per-object (PC/test_pair_counters/test_npairs_per_object_3d.py:60-91 draw 100 x 90 points)
+    p1, p2 = random_sample(100, seed=fixed_seed), random_sample(90, seed=fixed_seed + 7)
+    for per in (1.0, None):
+        if not np.array_equal(ORC.npairs_per_object_3d(p1, p2, [0.05, 0.1, 0.25], period=per),
+                              brute.npairs_per_object(p1, p2, [0.05, 0.1, 0.25], period=per)):
+            fails.append("per_object_%s" % per)
     w = np.random.RandomState(fixed_seed).random_sample(600)
     rbm = np.array([0.0, 0.1, 0.2, 0.3])
     for per in (1.0, None):
@@ -189,3 +205,15 @@ def test_planted_mistakes_are_caught(mutation):
     fails = known_answer_failures() + brute_force_failures()
     ORC.set_mutation(0)
     assert fails, "mutation %d went unnoticed" % mutation
+
+
+def test_per_object_rows_add_up_to_npairs_3d():
+    # every row set sums to the aggregate counts, whatever the mesh or threads
+    s1, s2 = random_sample(700, seed=11), random_sample(500, seed=12)
+    rb = np.array([0.02, 0.05, 0.1, 0.2, 0.3])
+    for per in (1.0, None):
+        tot = ORC.npairs_3d(s1, s2, rb, period=per)
+        for nt, cell in ((1, None), (4, 0.1), (3, [0.3, 0.1, 0.2])):
+            po = ORC.npairs_per_object_3d(s1, s2, rb, period=per, num_threads=nt, approx_cell1_size=cell)
+            assert po.shape == (700, 5) and np.array_equal(po.sum(axis=0), tot)
+            assert np.all(np.diff(po, axis=1) >= 0)
