@@ -236,6 +236,29 @@ def bench_other_configs(dev, steps):
     out["marked_npairs_3d_product_fwd_bwd"] = {
         "workload": "1M uniform halos, L=1000, 20 log edges 0.1-50, Zheng07 weights, d/d(logMmin, sigma)",
         "ms_per_fwd_bwd": s_fb * 1e3}
+
+    # marked (rp, pi) sums of the same halos, product of the Zheng07 weights
+    from halotools_amd.mock_observables import marked_npairs_xy_z
+    w4 = mocks.zheng07_ncen(logm, torch.tensor(12.0, dtype=torch.float64, device=dev),
+                            torch.tensor(0.3, dtype=torch.float64, device=dev)).contiguous()
+    rp4, pi4 = mocks.config_rbins(3), np.arange(0, 41, 1.0)
+    wx, s_mx = timed(lambda: marked_npairs_xy_z(t4, t4, rp4, pi4, period=1000.0, weights1=w4, weights2=w4,
+                                                weight_func_id=1))
+    cols4 = tuple(t4[:, i].contiguous() for i in range(3))
+    ev = _events(torch)
+    m4 = dict(n1=len(p4), n2=len(p4), period=[1000.0] * 3, pbc=True)
+    res = D.run(E.HP_MARKED_NPAIRS_XY_Z, [m4], cols4, cols4, rp4 * rp4, p2=pi4 * pi4, same=True,
+                w1=w4.view(-1, 1), nw=1, wfunc=1, want_counts=False, want_sums=True, want_evals=True, device=dev,
+                pair_events=(ev[0].cuda_event, ev[1].cuda_event))
+    torch.cuda.synchronize()
+    assert torch.equal(res.sums[0], wx)
+    ks = ev[0].elapsed_time(ev[1]) * 1e-3
+    evals = int(res.evals[0].item())
+    out["marked_npairs_xy_z"] = {
+        "workload": "1M uniform halos, L=1000, 16 log rp edges 0.1-30, pi edges 0..40 step 1, "
+                    "weight_func_id 1 with Zheng07 weights",
+        "ms_per_call": s_mx * 1e3, "pair_kernel_ms": ks * 1e3, "kernel_evals": evals,
+        "kernel_evals_per_s": evals / ks, "fp64_peak_frac": 8.0 * evals / ks / 1e12 / FP64_PEAK_TFLOPS}
     return out
 
 

@@ -110,7 +110,7 @@ def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0
     flags = 0
     if same:
         flags |= E.HP_SAME_SAMPLES
-    marked = kind == E.HP_MARKED_NPAIRS_3D
+    marked = kind in (E.HP_MARKED_NPAIRS_3D, E.HP_MARKED_NPAIRS_XY_Z)
     if marked:
         if rows1:
             flags |= E.HP_ROWS1
@@ -154,7 +154,8 @@ def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0
     else:
         res.counts = None
     req.status = res.status.data_ptr()
-    res.sums = torch.empty((M, nr), dtype=torch.float64, device=device) if (marked and want_sums) else None
+    sum_shape = (M, nr, len(p2)) if kind == E.HP_MARKED_NPAIRS_XY_Z else (M, nr)
+    res.sums = torch.empty(sum_shape, dtype=torch.float64, device=device) if (marked and want_sums) else None
     req.sums = res.sums.data_ptr() if res.sums is not None else None
     res.tangents = torch.empty((M, nr), dtype=torch.float64, device=device) if (marked and tangent) else None
     req.tangents = res.tangents.data_ptr() if res.tangents is not None else None
@@ -179,7 +180,8 @@ def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0
     req.tile_waves = int(tile_waves)
     res.fn = {E.HP_NPAIRS_3D: lib.hp_npairs_3d, E.HP_NPAIRS_PROJECTED: lib.hp_npairs_projected,
               E.HP_NPAIRS_XY_Z: lib.hp_npairs_xy_z, E.HP_MARKED_NPAIRS_3D: lib.hp_marked_npairs_3d,
-              E.HP_NPAIRS_PER_OBJECT_3D: lib.hp_npairs_per_object_3d}[kind]
+              E.HP_NPAIRS_PER_OBJECT_3D: lib.hp_npairs_per_object_3d,
+              E.HP_MARKED_NPAIRS_XY_Z: lib.hp_marked_npairs_xy_z}[kind]
     res.req = req
     res.device = device
     # keep every host and device buffer the request points to alive with it

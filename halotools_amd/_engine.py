@@ -18,6 +18,7 @@ HP_NPAIRS_PROJECTED = 1
 HP_NPAIRS_XY_Z = 2
 HP_MARKED_NPAIRS_3D = 3
 HP_NPAIRS_PER_OBJECT_3D = 4
+HP_MARKED_NPAIRS_XY_Z = 5
 
 HP_SAME_SAMPLES = 1
 HP_ROWS1 = 2
@@ -29,7 +30,7 @@ HP_TANGENT = 32
 EXPORTED_SYMBOLS = (
     "hp_last_error", "hp_abi_version", "hp_workspace_size", "hp_npairs_3d",
     "hp_npairs_projected", "hp_npairs_xy_z", "hp_marked_npairs_3d", "hp_npairs_per_object_3d",
-    "hp_marked_backward",
+    "hp_marked_npairs_xy_z", "hp_marked_backward",
     "hp_segment_sum_scratch", "hp_segment_sum", "hp_debug_bin_lookup", "hp_debug_hilbert",
 )
 
@@ -99,7 +100,7 @@ def library():
     lib.hp_workspace_size.argtypes = [ctypes.POINTER(HpRequest)]
     lib.hp_workspace_size.restype = ctypes.c_size_t
     for name in ("hp_npairs_3d", "hp_npairs_projected", "hp_npairs_xy_z", "hp_marked_npairs_3d",
-                 "hp_npairs_per_object_3d"):
+                 "hp_npairs_per_object_3d", "hp_marked_npairs_xy_z"):
         fn = getattr(lib, name)
         fn.argtypes = [ctypes.POINTER(HpRequest)]
         fn.restype = ctypes.c_int

@@ -17,8 +17,8 @@
 #include "hp_internal.h"
 
 namespace hp {
-int launch_reduce_sums_s(const double* partial, int nr, const MockGeom* geom, int nmocks, int side, int64_t max_groups,
-                         double* scratch, double* sums, hipStream_t s);
+int launch_reduce_sums_s(const double* partial, int nr, int np, const MockGeom* geom, int nmocks, int side,
+                         int64_t max_groups, double* scratch, double* sums, hipStream_t s);
 int launch_reduce_evals_s(const uint64_t* pevals, const MockGeom* geom, int nmocks, int side, int64_t max_groups,
                           uint64_t* scratch, int64_t* evals, hipStream_t s);
 int64_t reduce_scratch_elems(int64_t max_groups_per_mock, int nmocks, int nb);
@@ -149,6 +149,7 @@ struct Plan {
     int crep = 1;
     bool tangent = false;
     bool obj = false;  // per-object counts
+    bool mxyz = false;  // marked_npairs_xy_z: 2-D weighted sums
     size_t o_ptan = 0;
     size_t total = 0;
     std::vector<unsigned char> consts;  // bytes [0, o_gm[0])
@@ -174,7 +175,8 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     if (P.M < 1) return fail("nmocks must be >= 1");
     if (!r->mocks) return fail("mocks must not be null");
     P.same = (r->flags & HP_SAME_SAMPLES) != 0;
-    P.marked = (kind == HP_MARKED_NPAIRS_3D);
+    P.mxyz = (kind == HP_MARKED_NPAIRS_XY_Z);
+    P.marked = (kind == HP_MARKED_NPAIRS_3D) || P.mxyz;
     P.obj = (kind == HP_NPAIRS_PER_OBJECT_3D);
     P.rows1 = P.marked && (r->flags & HP_ROWS1);
     P.rows2 = P.marked && (r->flags & HP_ROWS2);
@@ -185,6 +187,9 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     if (r->tile_waves != 0 && r->tile_waves != 1 && r->tile_waves != 2 && r->tile_waves != 4 && r->tile_waves != 8)
         return fail("tile_waves must be 0, 1, 2, 4 or 8");
     if (P.marked) {
+        if (P.mxyz && (r->flags & ~HP_SAME_SAMPLES))
+            return fail("marked_npairs_xy_z takes no flags besides HP_SAME_SAMPLES");
+        if (P.mxyz && (r->wfunc < 1 || r->wfunc > 15)) return fail("marked_npairs_xy_z: weight_func_id must be in 1..15");
         if (r->wfunc < 1 || r->wfunc > 17) return fail("weight_func_id must be in 1..17");
         if (P.nw < 1 || P.nw > 5) return fail("weights per point must be in 1..5");
         if ((P.rows1 || P.rows2) && (r->wfunc != 1 || P.nw != 1))
@@ -195,20 +200,24 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     P.Kr = r->nr;
     if (P.Kr < 1 || P.Kr > HP_MAX_BINS) return fail("number of radial bins out of range");
     if (!r->r2) return fail("r2 must not be null");
-    if (kind == HP_NPAIRS_XY_Z || kind == HP_NPAIRS_PROJECTED) {
+    const bool xyz = (kind == HP_NPAIRS_XY_Z) || P.mxyz;  // 2-D (rp, pi) histogram
+    if (xyz || kind == HP_NPAIRS_PROJECTED) {
         P.Kp = r->np;
         if (P.Kp < 1 || P.Kp > HP_MAX_BINS || !r->p2) return fail("pi bins out of range");
         if (kind == HP_NPAIRS_PROJECTED && P.Kp != 1) return fail("npairs_projected takes one pi_max");
     }
     if (kind == HP_NPAIRS_XY_Z && P.Kr * P.Kp > 16384) return fail("rp_bins x pi_bins exceeds 16384 cells");
     if (P.marked && P.Kr > 128) return fail("marked counts support at most 128 bins");
+    // one FP64 (rp, pi) histogram per wavefront in LDS, with a trash row and column
+    if (P.mxyz && (P.Kr + 1) * (P.Kp + 1) > 5800)
+        return fail("marked_npairs_xy_z supports at most 5800 (len(rp_bins)+1)*(len(pi_bins)+1) cells");
 
     std::vector<double> er = suffix_min(r->r2, P.Kr);
     P.tr = make_table(er, P.Kr);
     double Rxy = sqrt(er[(size_t)P.Kr - 1]);
     double Rz = Rxy;
     int shape = 0;
-    if (kind == HP_NPAIRS_XY_Z) {
+    if (xyz) {
         std::vector<double> ep = suffix_min(r->p2, P.Kp);
         P.tp = make_table(ep, P.Kp);
         Rz = sqrt(ep[(size_t)P.Kp - 1]);
@@ -228,7 +237,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
         P.tp.t.assign(1, TabEntry{INFINITY, 0u, 0u});
     }
     if (!(Rxy == Rxy) || !(Rz == Rz)) return fail("bins must not be NaN");
-    P.nb_count = (kind == HP_NPAIRS_XY_Z) ? P.Kr * P.Kp : P.Kr;
+    P.nb_count = xyz ? P.Kr * P.Kp : P.Kr;
 
     // ---- per-mock geometry
     P.geom.resize((size_t)P.M);
@@ -366,11 +375,11 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     P.crep = (int)std::max<int64_t>(
         1, std::min<int64_t>({(int64_t)kCountReplicas, P.max_groups[0], (int64_t)(2048 / std::max(P.nb_count, 1))}));
     if (P.counts) P.o_pcount = take((size_t)P.M * (size_t)P.crep * (size_t)P.nb_count * 8);
-    if (P.sums) P.o_psum = take((size_t)gA * (size_t)P.Kr * 8);
+    if (P.sums) P.o_psum = take((size_t)gA * (size_t)P.nb_count * 8);
     if (P.tangent) P.o_ptan = take((size_t)gA * (size_t)P.Kr * 8);
     P.o_pevals = take((size_t)gA * 8);
     const int64_t red = reduce_scratch_elems(std::max<int64_t>(P.max_groups[0], 1), P.M,
-                                             P.Kr);  // sums (Kr) and evals (1)
+                                             P.nb_count);  // sums (Kr or Kr * Kp) and evals (1)
     P.o_red = take((size_t)red * 8);
     // auto-count with shared weights: rows2 = rows1 + seam corrections, no swapped pass
     P.elim = P.rows2 && P.same && P.marked && r->wfunc == 1;
@@ -454,7 +463,8 @@ static int run(const hp_request* r, int kind) {
     sb.vals[1] = reinterpret_cast<uint32_t*>(ws + P.o_vals[1]);
     sb.tile_counts = reinterpret_cast<uint32_t*>(ws + P.o_tiles);
     sb.digit_totals = reinterpret_cast<uint32_t*>(ws + P.o_digits);
-    if (r->build_events[0]) hipEventRecord(static_cast<hipEvent_t>(r->build_events[0]), s);
+    if (r->build_events[0] && hipEventRecord(static_cast<hipEvent_t>(r->build_events[0]), s) != hipSuccess)
+        return fail("event record failed");
     for (int sd = 0; sd < 2; sd++) {
         for (int o = 0; o < 2; o++) {
             const Side& S = P.sorted[sd][o];
@@ -475,7 +485,8 @@ static int run(const hp_request* r, int kind) {
             return fail("rank launch failed");
         hrank = hr;
     }
-    if (r->build_events[1]) hipEventRecord(static_cast<hipEvent_t>(r->build_events[1]), s);
+    if (r->build_events[1] && hipEventRecord(static_cast<hipEvent_t>(r->build_events[1]), s) != hipSuccess)
+        return fail("event record failed");
 
     const int s2 = P.same ? 0 : 1;
     int32_t* gm0 = reinterpret_cast<int32_t*>(ws + P.o_gm[0]);
@@ -507,6 +518,7 @@ static int run(const hp_request* r, int kind) {
             L.mode = kModeObj;
             L.orows = r->counts;
             break;
+        case HP_MARKED_NPAIRS_XY_Z: L.mode = kModeWXYZ; break;
         default:
             L.mode = P.tangent ? kModeTan
                                : ((r->wfunc == 1 && P.nw == 1 && (P.rows1 || P.rows2)) ? kModeJac : kModeWSum);
@@ -525,7 +537,8 @@ static int run(const hp_request* r, int kind) {
     if (P.counts)
         launch_fill_u32(reinterpret_cast<uint32_t*>(L.pcount), 0u, (int64_t)P.M * P.crep * P.nb_count * 2, s);
     int rc = 0;
-    if (r->pair_events[0]) hipEventRecord(static_cast<hipEvent_t>(r->pair_events[0]), s);
+    if (r->pair_events[0] && hipEventRecord(static_cast<hipEvent_t>(r->pair_events[0]), s) != hipSuccess)
+        return fail("event record failed");
     if (!(P.marked && !P.sums && !P.counts && !P.rows1 && !P.elim)) {
         rc = launch_pairs(L, s);
         if (rc == -2) return fail("histogram does not fit in LDS (too many bins)");
@@ -554,19 +567,20 @@ static int run(const hp_request* r, int kind) {
         // with SAME_SAMPLES the side-1 rows of the geometry equal the side-0 rows
         if (launch_pairs(B, s)) return fail("pair kernel (swapped) launch failed");
     }
-    if (r->pair_events[1]) hipEventRecord(static_cast<hipEvent_t>(r->pair_events[1]), s);
+    if (r->pair_events[1] && hipEventRecord(static_cast<hipEvent_t>(r->pair_events[1]), s) != hipSuccess)
+        return fail("event record failed");
     uint64_t* red = reinterpret_cast<uint64_t*>(ws + P.o_red);
     if (P.counts) {
         if (launch_prefix_counts(L.pcount, P.Kr, kind == HP_NPAIRS_XY_Z ? P.Kp : 0, P.M, P.crep, r->counts, s))
             return fail("reduction launch failed");
     }
     if (P.sums) {
-        if (launch_reduce_sums_s(L.psum, P.Kr, dgeom, P.M, 0, P.max_groups[0], reinterpret_cast<double*>(red),
-                                 r->sums, s))
+        if (launch_reduce_sums_s(L.psum, P.Kr, P.mxyz ? P.Kp : 0, dgeom, P.M, 0, P.max_groups[0],
+                                 reinterpret_cast<double*>(red), r->sums, s))
             return fail("reduction launch failed");
     }
     if (P.tangent) {
-        if (launch_reduce_sums_s(L.ptan, P.Kr, dgeom, P.M, 0, P.max_groups[0], reinterpret_cast<double*>(red),
+        if (launch_reduce_sums_s(L.ptan, P.Kr, 0, dgeom, P.M, 0, P.max_groups[0], reinterpret_cast<double*>(red),
                                  r->tangents, s))
             return fail("reduction launch failed");
     }
@@ -597,6 +611,7 @@ int hp_npairs_projected(const hp_request* req) { return run(req, HP_NPAIRS_PROJE
 int hp_npairs_xy_z(const hp_request* req) { return run(req, HP_NPAIRS_XY_Z); }
 int hp_marked_npairs_3d(const hp_request* req) { return run(req, HP_MARKED_NPAIRS_3D); }
 int hp_npairs_per_object_3d(const hp_request* req) { return run(req, HP_NPAIRS_PER_OBJECT_3D); }
+int hp_marked_npairs_xy_z(const hp_request* req) { return run(req, HP_MARKED_NPAIRS_XY_Z); }
 
 int hp_marked_backward(const double* rows, int32_t nr, const double* gsuffix, const int64_t* offsets,
                        int32_t nmocks, double* grad, void* stream) {

@@ -122,7 +122,7 @@ struct PairLaunch {
     const uint32_t* hrank;
 };
 enum Mode : int { kModeCount3D = 0, kModeProj = 1, kModeXYZ = 2, kModeWSum = 3, kModeJac = 4, kModeTan = 5,
-                  kModeObj = 6 };
+                  kModeObj = 6, kModeWXYZ = 7 };
 int launch_pairs(const PairLaunch& L, hipStream_t s);
 
 // reduce.hip

@@ -1,7 +1,8 @@
 // pairs.hip -- cell-pair tile kernels (replace the j-loops of
 // PC/cpairs/npairs_3d_engine.pyx:103-182, PC/cpairs/npairs_projected_engine.pyx:103-190,
-// PC/cpairs/npairs_xy_z_engine.pyx:103-194 and
-// PC/marked_cpairs/marked_npairs_3d_engine.pyx:103-216).
+// PC/cpairs/npairs_xy_z_engine.pyx:103-194,
+// PC/marked_cpairs/marked_npairs_3d_engine.pyx:103-216 and
+// PC/marked_cpairs/marked_npairs_xy_z_engine.pyx:103-226).
 //
 // Work decomposition (MI355X-first, not the reference's cell loop):
 //   * one wavefront = one "group" of 64 lane points that are consecutive in the
@@ -175,13 +176,18 @@ static_assert(kRing >= kBatch - 1 + 64, "survivor ring too small for the batch s
 // Occupancy the register allocation is held to (waves per SIMD): the count
 // kernels are limited by VGPRs otherwise, the FP64-histogram kernels by LDS.
 template <int MODE>
-constexpr int kWavesPerSimd = (MODE == kModeJac) ? 1 : ((MODE == kModeWSum || MODE == kModeTan) ? 4
-                                                                                 : (MODE == kModeXYZ ? 5 : 6));
+constexpr int kWavesPerSimd =
+    (MODE == kModeJac) ? 1
+                       : ((MODE == kModeWSum || MODE == kModeTan || MODE == kModeWXYZ) ? 4 : (MODE == kModeXYZ ? 5 : 6));
 // per-object counts: one u32 row set per lane (each lane owns its point's counts)
 
 template <int MODE, int SWAP, int ALSO, int MULTI, int WAVES>
 __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const PairArgs A) {
-    constexpr bool F64 = (MODE == kModeWSum || MODE == kModeJac || MODE == kModeTan);
+    constexpr bool F64 = (MODE == kModeWSum || MODE == kModeJac || MODE == kModeTan || MODE == kModeWXYZ);
+    // 2-D (rp, pi) binning: one histogram per wave, [Kr + 1][Kp + 1] cells
+    // (u32 counts, or FP64 weighted sums), the last row / column being trash
+    constexpr bool XY = (MODE == kModeXYZ || MODE == kModeWXYZ);
+    constexpr uint32_t cell_bytes = (MODE == kModeWXYZ) ? 8u : 4u;
     // LDS: the bucket tables (sized to this request's buckets), then one area per wave
     extern __shared__ __align__(16) unsigned char lds[];
     LdsEntry* const s_tr = reinterpret_cast<LdsEntry*>(lds);
@@ -203,12 +209,11 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     // 4-byte LDS operation is served 32 lanes per pass).
     constexpr bool SHARE4 = (MODE == kModeWSum || MODE == kModeTan);
     constexpr bool SHARE2 = (MODE == kModeCount3D || MODE == kModeProj);
-    const bool shared = (MODE == kModeXYZ) || (!F64 && A.shared_count);
-    const uint32_t step_r = (MODE == kModeXYZ)
-                                ? (uint32_t)(Kp + 1) * 4u
-                                : (shared ? 4u : ((SHARE4 || SHARE2) ? 128u : (MODE == kModeObj ? 256u : 512u)));
+    const bool shared = (MODE == kModeXYZ) || (!F64 && A.shared_count);  // shared u32 counts
+    const uint32_t step_r = XY ? (uint32_t)(Kp + 1) * cell_bytes
+                               : (shared ? 4u : ((SHARE4 || SHARE2) ? 128u : (MODE == kModeObj ? 256u : 512u)));
     const uint32_t lane_off =
-        shared ? 0u
+        (shared || XY) ? 0u
                : (SHARE4 ? (uint32_t)(lane & 15) * 8u
                          : (SHARE2 ? (uint32_t)(lane & 31) * 4u : (uint32_t)lane * (MODE == kModeObj ? 4u : 8u)));
     const uint32_t cnt_col = SHARE4 ? (uint32_t)(lane & 31) * 4u : 0u;  // SHARE4 count rows
@@ -221,12 +226,12 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
             s_tr[i].addr = e.k * step_r;
             s_tr[i].k = e.k;
         }
-        if (MODE == kModeXYZ) {
+        if (XY) {
             const TabEntry* gtp = reinterpret_cast<const TabEntry*>(ws + L.bp.table_off);
             for (int i = threadIdx.x; i < L.bp.nb; i += blockDim.x) {
                 const TabEntry e = gtp[i];
                 s_tp[i].edge = e.edge;
-                s_tp[i].addr = e.k * 4u;
+                s_tp[i].addr = e.k * cell_bytes;
                 s_tp[i].k = e.k;
             }
         }
@@ -308,7 +313,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     const uint32_t shr = L.br.shift, bshr = L.br.basesh, nbr1 = L.br.nb - 1;
     const uint32_t shp = L.bp.shift, bshp = L.bp.basesh, nbp1 = L.bp.nb - 1;
     const uint32_t trash_r = (uint32_t)Kr * step_r;  // projected: pairs beyond pi_max
-    const uint32_t trash_p = (uint32_t)Kp * 4u;      // xy_z: beyond the last pi edge
+    const uint32_t trash_p = (uint32_t)Kp * cell_bytes;  // xy_z: beyond the last pi edge
 
     uint64_t evals = 0;
     // Shared (per-wave) uint32 histograms can receive up to 64 increments per
@@ -349,14 +354,14 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
             const double dz2 = dz * dz;
             const uint32_t a = bin_addr<MULTI>(dxy, s_tr, shr, bshr, nbr1, step_r, gedges_r);
             return (dz2 <= pimax2) ? a : trash_r;
-        } else if (MODE == kModeXYZ) {
+        } else if (XY) {
             // pairs outside (rp_max, pi_max) are marked trash_r and skip the
             // atomics: in one shared per-wave histogram they would all pile
             // onto the same trash cells
             const double dxy = dx * dx + dy * dy;
             const double dz2 = dz * dz;
             const uint32_t ar = bin_addr<MULTI>(dxy, s_tr, shr, bshr, nbr1, step_r, gedges_r);
-            const uint32_t ap = bin_addr<MULTI>(dz2, s_tp, shp, bshp, nbp1, 4u, gedges_p);
+            const uint32_t ap = bin_addr<MULTI>(dz2, s_tp, shp, bshp, nbp1, cell_bytes, gedges_p);
             return (ar < trash_r && ap < trash_p) ? ar + ap : trash_r;
         } else {
             const double dsq = dx * dx + dy * dy + dz * dz;
@@ -513,7 +518,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
         } else {
 #pragma unroll
             for (int u = 0; u < B; u++)
-                if (MODE != kModeXYZ || a[u] != trash_r) accumulate(a[u], off[u] >> 3, rinc, false);
+                if (!XY || a[u] != trash_r) accumulate(a[u], off[u] >> 3, rinc, false);
         }
     };
     auto drain = [&]() __attribute__((always_inline)) {
@@ -686,7 +691,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                         __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(wsm + A.cnt_off + a[u] + cnt_col), inc,
                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
-                } else if (MODE != kModeTan && (MODE != kModeXYZ || a[u] != trash_r)) {
+                } else if (MODE != kModeTan && (!XY || a[u] != trash_r)) {
                     accumulate(a[u], 0u, inc, false);
                 }
             }
@@ -708,6 +713,11 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
         }
     }
     const double* fh = reinterpret_cast<const double*>(wsm);
+    if (MODE == kModeWXYZ && L.psum) {
+        // the group's (rp, pi) cells without the trash row and column
+        const int nc = Kr * Kp;
+        for (int s = lane; s < nc; s += 64) L.psum[g * nc + s] = fh[(s / Kp) * (Kp + 1) + s % Kp];
+    }
     if (MODE == kModeWSum) {
         if (L.psum) {
             for (int s = 0; s < Kr; s++) {
@@ -793,6 +803,7 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
             else wave_bytes = (Kr + 1) * 32 * 4;  // rows shared by lanes l and l+32
             break;
         case kModeXYZ: wave_bytes = (Kr + 1) * (Kp + 1) * 4; break;
+        case kModeWXYZ: wave_bytes = (Kr + 1) * (Kp + 1) * 8; break;
         case kModeObj: wave_bytes = (Kr + 1) * 64 * 4; break;
         case kModeJac:
             A.cnt_off = (Kr + 1) * 64 * 8;
@@ -812,7 +823,8 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
     if (L.br.nb > kTableMax || L.bp.nb > kTableMax) return -1;
     A.ring_off = align16(wave_bytes);
     A.wave_bytes = align16(A.ring_off + kRing * 4);
-    A.tab_bytes = align16((int)sizeof(LdsEntry) * (L.br.nb + (L.mode == kModeXYZ ? L.bp.nb : 0)));
+    const bool xy = (L.mode == kModeXYZ || L.mode == kModeWXYZ);
+    A.tab_bytes = align16((int)sizeof(LdsEntry) * (L.br.nb + (xy ? L.bp.nb : 0)));
     // Workgroup shape: the number of waves per workgroup that keeps the most
     // waves resident per CU (160 KB of LDS; register-limited waves per CU as
     // the kernels' launch bounds hold them); on ties 4 waves, then 8 (4 was
@@ -828,8 +840,10 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
             lds_per_block = 64 * 1024;
     }
     const int kLdsPerBlock = lds_per_block;
-    const int reg_waves = 4 * (L.mode == kModeJac ? 6 : ((L.mode == kModeWSum || L.mode == kModeTan) ? 4
-                                                          : (L.mode == kModeXYZ ? 5 : 6)));
+    const int reg_waves =
+        4 * (L.mode == kModeJac ? 6
+                                : ((L.mode == kModeWSum || L.mode == kModeTan || L.mode == kModeWXYZ) ? 4
+                                                                                                      : (L.mode == kModeXYZ ? 5 : 6)));
     int waves = 0, best = -1;
     if (L.waves > 0) {
         waves = L.waves;
@@ -851,6 +865,7 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
         case kModeProj: return launch_m<kModeProj, 0, 0>(A, waves, multi, s);
         case kModeXYZ: return launch_m<kModeXYZ, 0, 0>(A, waves, multi, s);
         case kModeObj: return launch_m<kModeObj, 0, 0>(A, waves, multi, s);
+        case kModeWXYZ: return launch_m<kModeWXYZ, 0, 0>(A, waves, multi, s);
         case kModeWSum:
             return also ? launch_m<kModeWSum, 0, 1>(A, waves, multi, s) : launch_m<kModeWSum, 0, 0>(A, waves, multi, s);
         case kModeJac:

@@ -139,11 +139,13 @@ int64_t reduce_scratch_elems(int64_t max_groups_per_mock, int nmocks, int nb) {
     return nc * nmocks * (int64_t)nb;
 }
 
-int launch_reduce_sums_s(const double* partial, int nr, const MockGeom* geom, int nmocks, int side, int64_t max_groups,
-                         double* scratch, double* sums, hipStream_t s) {
+// FP64 partial histograms [groups][nr] (2-D: [groups][nr * np]) -> cumulative sums per mock
+int launch_reduce_sums_s(const double* partial, int nr, int np, const MockGeom* geom, int nmocks, int side,
+                         int64_t max_groups, double* scratch, double* sums, hipStream_t s) {
     const int nc = (int)(nchunk_of(max_groups) > 0 ? nchunk_of(max_groups) : 1);
-    hipLaunchKernelGGL(k_reduce_s1<double>, dim3(nc, nmocks), dim3(256), 0, s, partial, nr, geom, side, nc, scratch);
-    hipLaunchKernelGGL((k_reduce_s2<double, double>), dim3(nmocks), dim3(256), 0, s, scratch, nr, 0, geom, side, nc,
+    const int nb = np > 0 ? nr * np : nr;
+    hipLaunchKernelGGL(k_reduce_s1<double>, dim3(nc, nmocks), dim3(256), 0, s, partial, nb, geom, side, nc, scratch);
+    hipLaunchKernelGGL((k_reduce_s2<double, double>), dim3(nmocks), dim3(256), 0, s, scratch, nr, np, geom, side, nc,
                        sums);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

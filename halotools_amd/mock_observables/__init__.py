@@ -1,6 +1,7 @@
 """Pair counters with the interface of ``halotools.mock_observables``
 (halotools/mock_observables/__init__.py:14-15 re-exports the same names)."""
-from .pair_counters import (marked_npairs_3d, npairs_3d, npairs_per_object_3d, npairs_projected,  # noqa: F401
-                            npairs_xy_z)
+from .pair_counters import (marked_npairs_3d, marked_npairs_xy_z, npairs_3d, npairs_per_object_3d,  # noqa: F401
+                            npairs_projected, npairs_xy_z)
 
-__all__ = ("npairs_3d", "npairs_projected", "npairs_xy_z", "marked_npairs_3d", "npairs_per_object_3d")
+__all__ = ("npairs_3d", "npairs_projected", "npairs_xy_z", "marked_npairs_3d", "marked_npairs_xy_z",
+           "npairs_per_object_3d")

@@ -6,6 +6,7 @@ The engine (PC/marked_cpairs/marked_npairs_3d_engine.pyx) and the marking
 functions (PC/marked_cpairs/marking_functions.pyx:14-217) are libhalopair.so.
 """
 import numpy as np
+import torch
 
 from ... import _device as D
 from ... import _engine as E
@@ -36,7 +37,12 @@ def _process_one(sample, weights, correct_shape, name, weight_func_id, correct_n
     converted = False
     if weights is None:
         return np.ones(correct_shape, dtype=np.float64)
-    weights = np.atleast_1d(weights).astype("float64")
+    if D.is_device_tensor(weights):
+        # device weights stay on the device (same checks and messages)
+        weights = weights.to(torch.float64)
+        weights = weights.reshape(1) if weights.ndim == 0 else weights
+    else:
+        weights = np.atleast_1d(weights).astype("float64")
     if weights.ndim == 1:
         converted = True
         weights = weights.reshape((len(weights), 1))
@@ -45,8 +51,8 @@ def _process_one(sample, weights, correct_shape, name, weight_func_id, correct_n
                "for the input `%s`. Instead, an array of \n"
                "dimension %%i was received." % name)
         raise HalotoolsError(msg % weights.ndim)
-    npts_weights, num_weights = np.shape(weights)[0], np.shape(weights)[1]
-    if np.shape(weights) != correct_shape:
+    npts_weights, num_weights = tuple(weights.shape)[0], tuple(weights.shape)[1]
+    if tuple(weights.shape) != correct_shape:
         if converted:
             msg = ("\n You passed in a 1-D array for `%s` that \n"
                    "does not have the correct length. The number of \n"
@@ -62,9 +68,10 @@ def _process_one(sample, weights, correct_shape, name, weight_func_id, correct_n
     return weights
 
 
-def _marked_npairs_process_weights(sample1, sample2, weights1, weights2, weight_func_id):
-    """PC/marked_npairs_3d.py:186-278."""
-    correct_num_weights = _func_signature_int_from_wfunc(weight_func_id)
+def _marked_npairs_process_weights(sample1, sample2, weights1, weights2, weight_func_id,
+                                   signature=_func_signature_int_from_wfunc):
+    """PC/marked_npairs_3d.py:186-278 (PC/marked_npairs_xy_z.py:197-289 with its own `signature`)."""
+    correct_num_weights = signature(weight_func_id)
     npts_sample1 = np.shape(sample1)[0]
     npts_sample2 = np.shape(sample2)[0]
     w1 = _process_one(sample1, weights1, (npts_sample1, correct_num_weights), "weights1", weight_func_id,
@@ -72,6 +79,17 @@ def _marked_npairs_process_weights(sample1, sample2, weights1, weights2, weight_
     w2 = _process_one(sample2, weights2, (npts_sample2, correct_num_weights), "weights2", weight_func_id,
                       correct_num_weights)
     return w1, w2
+
+
+def same_weights(w1, w2):
+    """True when both weight arrays hold the same values (NumPy or device tensors)."""
+    if w1 is w2:
+        return True
+    if isinstance(w1, torch.Tensor) or isinstance(w2, torch.Tensor):
+        if not (isinstance(w1, torch.Tensor) and isinstance(w2, torch.Tensor)) or w1.device != w2.device:
+            return False
+        return w1.shape == w2.shape and bool(torch.equal(w1, w2))
+    return np.array_equal(w1, w2)
 
 
 def marked_npairs_3d(sample1, sample2, rbins, weight_func_id,
@@ -109,7 +127,7 @@ def marked_npairs_3d(sample1, sample2, rbins, weight_func_id,
 
     device = D.require_gpu(sample1.device if D.is_device_tensor(sample1) else None)
     as_tensor = D.is_device_tensor(sample1)
-    same = D.same_array(sample1, sample2) and np.array_equal(weights1, weights2)
+    same = D.same_array(sample1, sample2) and same_weights(weights1, weights2)
     s1, s2 = C.upload((x1, y1, z1), (x2, y2, z2), D.same_array(sample1, sample2), device)
     nw = weights1.shape[1]
     w1 = D.to_device_weights(weights1, device)

@@ -3,7 +3,7 @@
  *
  * Drop-in replacement for the native engines behind
  * halotools.mock_observables.{npairs_3d, npairs_projected, npairs_xy_z,
- * marked_npairs_3d}.  The reference binds its engines as Cython `def`
+ * marked_npairs_3d, marked_npairs_xy_z, npairs_per_object_3d}.  The reference binds its engines as Cython `def`
  * functions that take a RectangularDoubleMesh plus the six coordinate arrays
  * and return a fresh NumPy array; here the mesh is built on the GPU and every
  * buffer is caller-owned device memory passed as a plain pointer.
@@ -49,7 +49,8 @@ enum hp_kind {
     HP_NPAIRS_PROJECTED = 1, /* PC/cpairs/npairs_projected_engine.pyx:17     */
     HP_NPAIRS_XY_Z = 2,      /* PC/cpairs/npairs_xy_z_engine.pyx:17          */
     HP_MARKED_NPAIRS_3D = 3, /* PC/marked_cpairs/marked_npairs_3d_engine.pyx:22 */
-    HP_NPAIRS_PER_OBJECT_3D = 4 /* PC/cpairs/npairs_per_object_3d_engine.pyx:19 */
+    HP_NPAIRS_PER_OBJECT_3D = 4, /* PC/cpairs/npairs_per_object_3d_engine.pyx:19 */
+    HP_MARKED_NPAIRS_XY_Z = 5 /* PC/marked_cpairs/marked_npairs_xy_z_engine.pyx:22 */
 };
 
 /* Bit flags of hp_request.flags */
@@ -78,7 +79,7 @@ typedef struct hp_request {
     int32_t kind;              /* enum hp_kind                                         */
     int32_t flags;             /* enum hp_flags                                        */
     int32_t nmocks;            /* >= 1                                                 */
-    int32_t wfunc;             /* marked: weight_func_id 1..17                         */
+    int32_t wfunc;             /* marked: weight_func_id 1..17 (xy_z: 1..15)           */
     const hp_mock* mocks;      /* host [nmocks]; sample points of mock m follow those  */
                                /* of mock m-1 in the coordinate arrays                 */
     const double *x1, *y1, *z1; /* sample 1, SoA, n1 total                              */
@@ -97,6 +98,7 @@ typedef struct hp_request {
     /* outputs, device */
     int64_t* counts;           /* [nmocks][nr] cumulative (xy_z: [nmocks][nr][np])      */
     double* sums;              /* marked: [nmocks][nr] cumulative weighted sums         */
+                               /* (marked xy_z: [nmocks][nr][np])                      */
     double* rows1;             /* HP_ROWS1: [n1_total][nr] per-bin (not cumulative)     */
     double* rows2;             /* HP_ROWS2: [n2_total][nr]                              */
     int64_t* evals;            /* optional [nmocks]: candidate pairs evaluated          */
@@ -147,6 +149,13 @@ int hp_npairs_per_object_3d(const hp_request* req);
  * point's weight: rows1[i][s] = sum of w2_j over partners j in bin s.  This is
  * the Jacobian the backward pass contracts (no counterpart in the reference). */
 int hp_marked_npairs_3d(const hp_request* req);
+
+/* marked_npairs_xy_z: sums[m][k][g] of wfunc(w1_i, w2_j) over pairs with
+ * dxy^2 <= r2[k] and dz^2 <= p2[g] (cumulative along both axes).  Replaces
+ * PC/marked_cpairs/marked_npairs_xy_z_engine.pyx:22-228 with the marking
+ * functions 1-15 it dispatches to (:231-263).  No flags beyond
+ * HP_SAME_SAMPLES. */
+int hp_marked_npairs_xy_z(const hp_request* req);
 
 /* Backward of the product-weighted marked sums (no counterpart in the
  * reference): grad[p] = sum_s rows[p][s] * gsuffix[mock(p)][s], where

@@ -122,3 +122,27 @@ def npairs_per_object(sample1, sample2, rbins, period=None):
         d = np.sqrt(np.sum(m * m, axis=1))
         out[i] = np.sum(d[:, None] <= rbins[None, :], axis=0)
     return out
+
+
+def xy_z_wnpairs_generic(sample1, sample2, rp_bins, pi_bins, weights1, weights2, wfunc, period=None):
+    """Brute-force marked (rp, pi) sums: PC/pairs.py:248-352 (xy_z_wnpairs, which
+    sums w1*w2 over pairs with rp <= rp_bins[i] and pi <= pi_bins[j]) with an
+    arbitrary marking function ``wfunc(w1_row, w2_row)``; math.fsum per cell."""
+    sample1 = np.atleast_2d(sample1)
+    sample2 = np.atleast_2d(sample2)
+    rp_bins = np.atleast_1d(rp_bins)
+    pi_bins = np.atleast_1d(pi_bins)
+    period = _period(period, sample1.shape[-1])
+    terms = [[[] for _ in range(pi_bins.size)] for _ in range(rp_bins.size)]
+    for i in range(len(sample1)):
+        m = _min_image(sample1[i], sample2, period)
+        rp = np.sqrt(np.sum(m[:, :-1] ** 2, axis=1))
+        pi = np.sqrt(m[:, -1] ** 2)
+        for j in np.nonzero((rp <= rp_bins.max()) & (pi <= pi_bins.max()))[0]:
+            w = wfunc(weights1[i], weights2[j])
+            for a in range(rp_bins.size):
+                if rp[j] <= rp_bins[a]:
+                    for b in range(pi_bins.size):
+                        if pi[j] <= pi_bins[b]:
+                            terms[a][b].append(w)
+    return np.array([[math.fsum(t) for t in row] for row in terms])

@@ -127,7 +127,7 @@ double ref_wfunc(int id, const double* w1, const double* w2) {
 /* The visitor is invoked for every (cell1, cell2) pair with the cell-2     */
 /* shift values, exactly where the Cython engines enter their i/j loops.    */
 
-enum { KIND_3D = 0, KIND_PROJ = 1, KIND_XYZ = 2, KIND_MARKED = 3, KIND_OBJ = 4 };
+enum { KIND_3D = 0, KIND_PROJ = 1, KIND_XYZ = 2, KIND_MARKED = 3, KIND_OBJ = 4, KIND_MARKED_XYZ = 5 };
 
 typedef struct ref_job {
     int kind;
@@ -222,6 +222,22 @@ static void run_cells(const ref_job* J, int64_t first, int64_t last) {
                                     k = k - 1;
                                     if (k < 0) break;
                                 }
+                            } else if (J->kind == KIND_MARKED_XYZ) {
+                                /* PC/marked_cpairs/marked_npairs_xy_z_engine.pyx:214-226 */
+                                const double dxy_sq = dx * dx + dy * dy;
+                                const double dz_sq = dz * dz;
+                                const double weight = wfunc_from_id(J->wfunc_id)(J->w1 + i * J->nw, J->w2 + j * J->nw);
+                                int k = K - 1;
+                                while (bin_pass(dxy_sq, J->r2[k])) {
+                                    int g = G - 1;
+                                    while (bin_pass(dz_sq, J->pi2[g])) {
+                                        J->wcounts[(int64_t)k * G + g] += weight;
+                                        g = g - 1;
+                                        if (g < 0) break;
+                                    }
+                                    k = k - 1;
+                                    if (k < 0) break;
+                                }
                             } else if (J->kind == KIND_PROJ) {
                                 /* PC/cpairs/npairs_projected_engine.pyx:182-190 */
                                 const double dxy_sq = dx * dx + dy * dy;
@@ -276,7 +292,8 @@ static int run_threads(const ref_job* J, int nthreads) {
     const ref_mesh* m = J->m;
     const int64_t first = m->first_cell1, last = m->last_cell1;
     const int64_t ncells = last - first;
-    const int64_t outlen = (J->kind == KIND_XYZ) ? (int64_t)J->K * J->G : J->K;
+    const int64_t outlen = (J->kind == KIND_XYZ || J->kind == KIND_MARKED_XYZ) ? (int64_t)J->K * J->G : J->K;
+    const int fp_out = (J->kind == KIND_MARKED || J->kind == KIND_MARKED_XYZ);
     /* per-object rows of different cell-1 chunks are disjoint: workers share
      * the output (the reference sums per-worker arrays that are zero elsewhere) */
     const int shared_out = (J->kind == KIND_OBJ);
@@ -294,7 +311,7 @@ static int run_threads(const ref_job* J, int nthreads) {
     for (int t = 0; t < nthreads; t++) {
         const int64_t len = base + (t < extra ? 1 : 0);
         args[t].job = *J;
-        if (J->kind == KIND_MARKED) {
+        if (fp_out) {
             args[t].job.wcounts = (double*)calloc((size_t)outlen, sizeof(double));
         } else if (!shared_out) {
             args[t].job.counts = (int64_t*)calloc((size_t)outlen, sizeof(int64_t));
@@ -308,7 +325,7 @@ static int run_threads(const ref_job* J, int nthreads) {
     for (int t = 0; t < nthreads; t++) {
         if (shared_out) continue;
         for (int64_t k = 0; k < outlen; k++) {
-            if (J->kind == KIND_MARKED) J->wcounts[k] += args[t].job.wcounts[k];
+            if (fp_out) J->wcounts[k] += args[t].job.wcounts[k];
             else J->counts[k] += args[t].job.counts[k];
         }
         free(args[t].job.counts);
@@ -409,6 +426,27 @@ int ref_marked_npairs_3d(const ref_mesh* m, const double* x1, const double* y1, 
     memset(wcounts, 0, sizeof(double) * (size_t)K);
     int rc = run_threads(&J, nthreads);
     free(r2);
+    return rc;
+}
+
+int ref_marked_npairs_xy_z(const ref_mesh* m, const double* x1, const double* y1, const double* z1,
+                           const double* x2, const double* y2, const double* z2,
+                           const double* w1, const double* w2, int nw, int wfunc_id,
+                           const double* rp_bins, int K, const double* pi_bins, int G, int nthreads,
+                           double* wcounts) {
+    if (!wfunc_from_id(wfunc_id)) return -2;
+    ref_job J;
+    memset(&J, 0, sizeof(J));
+    J.kind = KIND_MARKED_XYZ; J.m = m;
+    J.x1 = x1; J.y1 = y1; J.z1 = z1; J.x2 = x2; J.y2 = y2; J.z2 = z2;
+    J.w1 = w1; J.w2 = w2; J.nw = nw; J.wfunc_id = wfunc_id;
+    double* r2 = square_bins(rp_bins, K);
+    double* p2 = square_bins(pi_bins, G);
+    J.r2 = r2; J.K = K; J.pi2 = p2; J.G = G; J.wcounts = wcounts;
+    memset(wcounts, 0, sizeof(double) * (size_t)K * (size_t)G);
+    int rc = run_threads(&J, nthreads);
+    free(r2);
+    free(p2);
     return rc;
 }
 

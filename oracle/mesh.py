@@ -74,6 +74,8 @@ def load(fast=False):
     lib.ref_npairs_projected.argtypes = [ctypes.POINTER(_RefMeshC), P, P, P, P, P, P, P, I, D, I, P]
     lib.ref_npairs_xy_z.argtypes = [ctypes.POINTER(_RefMeshC), P, P, P, P, P, P, P, I, P, I, I, P]
     lib.ref_marked_npairs_3d.argtypes = [ctypes.POINTER(_RefMeshC), P, P, P, P, P, P, P, P, I, I, P, I, I, P]
+    lib.ref_marked_npairs_xy_z.argtypes = [ctypes.POINTER(_RefMeshC), P, P, P, P, P, P, P, P, I, I, P, I, P, I, I,
+                                           P]
     lib.ref_npairs_per_object_3d.argtypes = [ctypes.POINTER(_RefMeshC), P, P, P, P, P, P, P, I, ctypes.c_int64, I, P]
     lib.ref_candidate_pairs.argtypes = [ctypes.POINTER(_RefMeshC)]
     lib.ref_candidate_pairs.restype = ctypes.c_int64
@@ -359,6 +361,32 @@ def marked_npairs_3d(sample1, sample2, rbins, period=None, weights1=None, weight
     rc = load(fast).ref_marked_npairs_3d(ctypes.byref(mc), *_ptrs(cols), w1s.ctypes.data, w2s.ctypes.data,
                                          w1s.shape[1], int(weight_func_id), rbins.ctypes.data, len(rbins),
                                          int(num_threads), out.ctypes.data)
+    assert rc == 0
+    return out
+
+
+def marked_npairs_xy_z(sample1, sample2, rp_bins, pi_bins, period=None, weights1=None, weights2=None,
+                       weight_func_id=0, num_threads=1, approx_cell1_size=None, approx_cell2_size=None,
+                       fast=False, nw=None):
+    """Oracle for PC/marked_npairs_xy_z.py:23-194: (len(rp_bins), len(pi_bins)) sums of
+    wfunc(w1_i, w2_j) (engine loop PC/marked_cpairs/marked_npairs_xy_z_engine.pyx:173-226)."""
+    rp_bins = np.atleast_1d(rp_bins).astype("f8")
+    pi_bins = np.atleast_1d(pi_bins).astype("f8")
+    rp_max, pi_max = np.max(rp_bins), np.max(pi_bins)
+    n1, n2 = len(sample1), len(sample2)
+    if nw is None:
+        nw = 1 if weights1 is None or np.ndim(weights1) == 1 else np.shape(weights1)[1]
+    w1 = np.ones((n1, nw)) if weights1 is None else np.asarray(weights1, dtype=np.float64).reshape(n1, -1)
+    w2 = np.ones((n2, nw)) if weights2 is None else np.asarray(weights2, dtype=np.float64).reshape(n2, -1)
+    mesh, cols = _prepare(sample1, sample2, (rp_max, rp_max, pi_max), period,
+                          approx_cell1_size, approx_cell2_size, (rp_max, rp_max, pi_max))
+    w1s = np.ascontiguousarray(w1[mesh.mesh1.idx_sorted, :])
+    w2s = np.ascontiguousarray(w2[mesh.mesh2.idx_sorted, :])
+    out = np.zeros((len(rp_bins), len(pi_bins)), dtype=np.float64)
+    mc = mesh.as_c()
+    rc = load(fast).ref_marked_npairs_xy_z(ctypes.byref(mc), *_ptrs(cols), w1s.ctypes.data, w2s.ctypes.data,
+                                           w1s.shape[1], int(weight_func_id), rp_bins.ctypes.data, len(rp_bins),
+                                           pi_bins.ctypes.data, len(pi_bins), int(num_threads), out.ctypes.data)
     assert rc == 0
     return out
 

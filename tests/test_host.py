@@ -13,7 +13,7 @@ import pytest
 
 from halotools_amd import _engine as E
 from halotools_amd.custom_exceptions import HalotoolsError
-from halotools_amd.mock_observables import marked_npairs_3d, npairs_3d, npairs_projected, npairs_xy_z
+from halotools_amd.mock_observables import marked_npairs_3d, marked_npairs_xy_z, npairs_3d, npairs_projected, npairs_xy_z
 from halotools_amd.mock_observables.pair_counters import _process as P
 from halotools_amd.mock_observables.pair_counters.marked_npairs_3d import _func_signature_int_from_wfunc
 from tests.helpers import fixed_seed, generate_locus_of_3d_points
@@ -184,6 +184,68 @@ def test_marked_wrong_weight_width_raises(wid):
     w = np.random.RandomState(fixed_seed).random_sample(200 * width).reshape(200, width) - 0.5
     with pytest.raises(HalotoolsError):
         marked_npairs_3d(s, s, np.array([0.0, 0.1, 0.2, 0.3]), wid, period=1.0, weights1=w, weights2=w)
+
+
+def test_marked_xy_z_wfunc_signatures():
+    # PC/test_pair_counters/test_marked_npairs_xy_z.py:194-248: ids 1-15, widths as for 3-D
+    from halotools_amd.mock_observables.pair_counters.marked_npairs_xy_z import \
+        _func_signature_int_from_wfunc as sig_xyz
+    assert [sig_xyz(i) for i in range(1, 16)] == [1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 4, 4, 3, 3]
+    for bad in (0, 16, 17):
+        with pytest.raises(HalotoolsError) as err:
+            sig_xyz(bad)
+        assert "The value ``weight_func_id`` = %i is not recognized" % bad in err.value.args[0]
+    with pytest.raises(ValueError):
+        sig_xyz(1.0)
+
+
+@pytest.mark.parametrize("wid", list(range(1, 16)))
+def test_marked_xy_z_wrong_weight_width_raises(wid):
+    rng = np.random.RandomState(fixed_seed)
+    s = rng.random_sample((200, 3))
+    width = _func_signature_int_from_wfunc(wid) + 1
+    w = np.random.RandomState(fixed_seed).random_sample(200 * width).reshape(200, width) - 0.5
+    with pytest.raises(HalotoolsError):
+        marked_npairs_xy_z(s, s, np.array([0.0, 0.1, 0.2, 0.3]), np.array([0, 0.15]), period=1.0, weights1=w,
+                           weights2=w, weight_func_id=wid)
+
+
+def test_marked_xy_z_default_wfunc_rejected():
+    # the reference's default weight_func_id=0 is not a marking function of the xy_z engine
+    s = np.random.RandomState(2).random_sample((50, 3))
+    with pytest.raises(HalotoolsError):
+        marked_npairs_xy_z(s, s, np.array([0.1, 0.2]), np.array([0.1, 0.2]), period=1.0)
+
+
+def _request(kind, flags=0, wfunc=1, nw=1, nr=4, np_=2):
+    mock = E.HpMock()
+    mock.n1, mock.n2 = 100, 100
+    mock.period[:] = [1.0, 1.0, 1.0]
+    mock.pbc = 1
+    r2 = (ctypes.c_double * nr)(*[(0.1 * (i + 1)) ** 2 / 4 for i in range(nr)])
+    p2 = (ctypes.c_double * np_)(*[(0.1 * (i + 1)) ** 2 for i in range(np_)])
+    req = E.HpRequest()
+    req.kind, req.flags, req.nmocks, req.wfunc, req.nw, req.nr, req.np = kind, flags, 1, wfunc, nw, nr, np_
+    req.mocks = ctypes.pointer(mock)
+    req.r2 = ctypes.cast(r2, ctypes.POINTER(ctypes.c_double))
+    req.p2 = ctypes.cast(p2, ctypes.POINTER(ctypes.c_double))
+    return req, (mock, r2, p2)
+
+
+def test_marked_xy_z_request_validation():
+    lib = E.library()
+    req, keep = _request(E.HP_MARKED_NPAIRS_XY_Z)
+    assert lib.hp_workspace_size(ctypes.byref(req)) > 0
+    for flags in (E.HP_ROWS1, E.HP_ALSO_COUNTS, E.HP_TANGENT, E.HP_NO_SUMS):
+        req, keep = _request(E.HP_MARKED_NPAIRS_XY_Z, flags=flags)
+        assert lib.hp_workspace_size(ctypes.byref(req)) == 0
+        assert "takes no flags" in lib.hp_last_error().decode()
+    req, keep = _request(E.HP_MARKED_NPAIRS_XY_Z, wfunc=16, nw=5)
+    assert lib.hp_workspace_size(ctypes.byref(req)) == 0
+    assert "1..15" in lib.hp_last_error().decode()
+    req, keep = _request(E.HP_MARKED_NPAIRS_XY_Z, nr=100, np_=100)
+    assert lib.hp_workspace_size(ctypes.byref(req)) == 0
+    assert "5800" in lib.hp_last_error().decode()
 
 
 def test_marked_weight_length_message():

@@ -64,6 +64,18 @@ def known_answer_failures():
     res = ORC.marked_npairs_3d(pts, pts, rbm, 1.0, np.ones(1000) * 3, np.ones(1000) * 3, 1)
     if not np.all(res == 9.0 * plain):
         fails.append("marked_uniform")
+    # marked (rp, pi), uniform weights (PC/test_pair_counters/test_marked_npairs_xy_z.py:251-285, 646-700)
+    pim = np.array([0, 0.15])
+    plain2 = brute.xy_z_npairs(pts, pts, rbm, pim, period=1.0)
+    res = ORC.marked_npairs_xy_z(pts, pts, rbm, pim, period=1.0, weights1=np.ones(1000) * 3,
+                                 weights2=np.ones(1000) * 3, weight_func_id=1)
+    if not (res.shape == (4, 2) and np.all(res == 9.0 * plain2)):
+        fails.append("marked_xyz_uniform")
+    w11 = np.tile((1.0, 0.0), 1000).reshape((1000, 2))
+    res = ORC.marked_npairs_xy_z(pts, pts, rbm, pim, period=1.0, weights1=w11,
+                                 weights2=np.tile((1.0, 2.0), 1000).reshape((1000, 2)), weight_func_id=11)
+    if not np.all(res == 2.0 * plain2):
+        fails.append("marked_xyz_id11")
     # per-object counts (PC/test_pair_counters/test_npairs_per_object_3d.py:19-57)
     g5 = generate_3d_regular_mesh(5)
     loc = generate_locus_of_3d_points(100, xc=0.101, yc=0.101, zc=0.101, seed=fixed_seed)
@@ -107,6 +119,14 @@ def brute_force_failures():
         b = brute.wnpairs(s, s, rbm, period=per, weights1=w, weights2=w)
         if not np.allclose(a, b, rtol=1e-13, atol=0):
             fails.append("marked_%s" % per)
+    # marked (rp, pi) with random weights (PC/test_pair_counters/test_marked_npairs_xy_z.py:62-133)
+    w1, w2 = np.random.RandomState(3).random_sample(200), np.random.RandomState(4).random_sample(150)
+    for per in (1.0, None):
+        a = ORC.marked_npairs_xy_z(s1, s2, rp, pi, period=per, weights1=w1, weights2=w2, weight_func_id=1)
+        b = brute.xy_z_wnpairs_generic(s1, s2, rp, pi, w1[:, None], w2[:, None], lambda u, v: u[0] * v[0],
+                                       period=per)
+        if not (a.shape == (len(rp), len(pi)) and np.allclose(a, b, rtol=1e-13, atol=0)):
+            fails.append("marked_xyz_%s" % per)
     return fails
 
 
@@ -145,6 +165,25 @@ def test_marking_function_values(wid, a, b, expected):
 
 
 # ----------------------------------------------------------- invariants
+@pytest.mark.parametrize("wid", list(range(1, 16)))
+def test_marked_xy_z_wfuncs_vs_brute_force(wid):
+    from halotools_amd.mock_observables.pair_counters.marked_npairs_3d import _WFUNC_WIDTH
+    width = _WFUNC_WIDTH[wid]
+    s1, s2 = random_sample(250, seed=20 + wid), random_sample(220, seed=40 + wid)
+    rp = np.array([0.0, 0.05, 0.1, 0.2, 0.3])
+    pi = np.array([0.02, 0.1, 0.25])
+    w1 = np.random.RandomState(wid).random_sample((250, width)) - 0.5
+    w2 = np.random.RandomState(wid + 50).random_sample((220, width)) - 0.5
+    w1[:, 0] = np.round(w1[:, 0] * 4) / 4  # ties for the comparison-based marks
+    w2[:, 0] = np.round(w2[:, 0] * 4) / 4
+    for per in (1.0, None):
+        a = ORC.marked_npairs_xy_z(s1, s2, rp, pi, period=per, weights1=w1, weights2=w2, weight_func_id=wid,
+                                   num_threads=3)
+        b = brute.xy_z_wnpairs_generic(s1, s2, rp, pi, w1, w2, lambda u, v: ORC.wfunc(wid, u, v), period=per)
+        scale = max(1.0, np.abs(b).max())
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-13 * scale)
+
+
 def test_mesh_independence_and_threads():
     s1, s2 = random_sample(1500, seed=3), random_sample(1200, seed=4)
     rb = np.array([0.01, 0.05, 0.1, 0.2])
