@@ -3,5 +3,7 @@
 from .pair_counters import (marked_npairs_3d, marked_npairs_xy_z, npairs_3d, npairs_per_object_3d,  # noqa: F401
                             npairs_projected, npairs_xy_z)
 
+from .two_point_clustering import marked_tpcf, rp_pi_tpcf, tpcf, wp  # noqa: F401
+
 __all__ = ("npairs_3d", "npairs_projected", "npairs_xy_z", "marked_npairs_3d", "marked_npairs_xy_z",
-           "npairs_per_object_3d")
+           "npairs_per_object_3d", "tpcf", "rp_pi_tpcf", "wp", "marked_tpcf")

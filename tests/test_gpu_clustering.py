@@ -1,0 +1,202 @@
+"""GPU parity of the two-point caller layer: tpcf, rp_pi_tpcf, wp and
+marked_tpcf on the device pair counters against the oracle
+(oracle/clustering.py over the C restatement's counts).
+
+Unweighted estimators must be bit-identical to the oracle: the counts are
+exact and the differencing and estimator arithmetic are the same IEEE
+operations in the same order.  Marked ratios agree to 1e-12 relative (the
+weighted sums are summed in another order).  The reference's own checks
+(TPC/tests/test_tpcf.py, test_rp_pi_tpcf.py, test_wp.py, test_marked_tpcf.py;
+TPC = halotools/mock_observables/two_point_clustering) are restated too.
+"""
+import warnings
+
+import numpy as np
+import pytest
+
+from oracle import clustering as OC
+from tests.helpers import fixed_seed, random_sample
+
+pytestmark = pytest.mark.gpu
+
+ESTIMATORS = ["Natural", "Davis-Peebles", "Hewett", "Hamilton", "Landy-Szalay"]
+PER = np.array([1.0, 1.0, 1.0])
+
+
+@pytest.fixture(scope="module")
+def mo(gpu):
+    import halotools_amd.mock_observables as mo
+    return mo
+
+
+def _samples(n1=1000, n2=900, nr=1500):
+    return random_sample(n1, seed=1), random_sample(n2, seed=2), random_sample(nr, seed=3)
+
+
+@pytest.mark.parametrize("est", ESTIMATORS)
+@pytest.mark.parametrize("period", [PER, None])
+def test_tpcf_auto_with_randoms_exact(mo, est, period):
+    s1, s2, r = _samples()
+    rb = np.linspace(0.01, 0.3, 8)
+    got = mo.tpcf(s1, rb, randoms=r, period=period, estimator=est)
+    ref = OC.tpcf(s1, rb, randoms=r, period=period, estimator_name=est)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("est", ["Natural", "Hamilton", "Landy-Szalay"])
+def test_tpcf_cross_with_randoms_exact(mo, est):
+    s1, s2, r = _samples()
+    rb = np.linspace(0.01, 0.3, 8)
+    got = mo.tpcf(s1, rb, sample2=s2, randoms=r, period=PER, estimator=est)
+    ref = OC.tpcf(s1, rb, sample2=s2, randoms=r, period=PER, estimator_name=est)
+    assert len(got) == 3
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("est", ESTIMATORS)
+def test_tpcf_analytic_randoms_exact(mo, est):
+    s1, s2, _ = _samples()
+    rb = np.logspace(-2, np.log10(0.3), 10)
+    assert np.array_equal(mo.tpcf(s1, rb, period=1.0, estimator=est),
+                          OC.tpcf(s1, rb, period=PER, estimator_name=est))
+
+
+def test_tpcf_return_shapes_and_consistency(mo):
+    # TPC/tests/test_tpcf.py:32-111, 247-455
+    s1, s2, r = _samples(100, 101, 102)
+    rb = np.linspace(0.001, 0.3, 5)
+    assert len(mo.tpcf(s1, rb, sample2=s2, randoms=r, period=None)) == 3
+    assert len(mo.tpcf(s1, rb, sample2=s2, randoms=None, period=PER)) == 3
+    assert len(mo.tpcf(s1, rb, sample2=s2, randoms=r, period=PER)) == 3
+    xi11, xi12, xi22 = mo.tpcf(s1, rb, sample2=s2, randoms=r, period=PER, estimator="Natural")
+    xi12b = mo.tpcf(s1, rb, sample2=s2, randoms=r, period=PER, estimator="Natural", do_auto=False)
+    xi11b, xi22b = mo.tpcf(s1, rb, sample2=s2, randoms=r, period=PER, estimator="Natural", do_cross=False)
+    assert np.array_equal(xi12, xi12b) and np.array_equal(xi11, xi11b) and np.array_equal(xi22, xi22b)
+    auto = mo.tpcf(s1, rb, randoms=r, period=PER)
+    assert np.array_equal(auto, xi11)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        same = mo.tpcf(s1, rb, sample2=s1.copy(), randoms=r, period=PER)
+        assert "exactly the same" in str(w[-1].message)
+    assert np.array_equal(same, auto)
+
+
+@pytest.mark.parametrize("est", ["Natural", "Landy-Szalay"])
+def test_rr_precomputed(mo, est):
+    # TPC/tests/test_tpcf.py:535-767: a precomputed RR reproduces the normal result
+    from halotools_amd.mock_observables.two_point_clustering.tpcf import _random_counts
+    s1, _, r = _samples(1000, 10, 100)
+    rb = np.linspace(0.001, 0.3, 5)
+    rmax = rb.max()
+    normal = mo.tpcf(s1, rb, randoms=r, period=PER, estimator=est, approx_cell1_size=[rmax] * 3,
+                     approx_cellran_size=[rmax] * 3)
+    D1R, D2R, RR = _random_counts(s1, s1, r, rb, PER, True, 1, True, True, True, [rmax] * 3, [rmax] * 3,
+                                  [rmax] * 3)
+    assert isinstance(RR, np.ndarray)
+    pre = mo.tpcf(s1, rb, randoms=r, period=PER, estimator=est, RR_precomputed=RR, NR_precomputed=len(r))
+    assert np.array_equal(pre, normal)
+
+
+@pytest.mark.parametrize("est", ESTIMATORS)
+@pytest.mark.parametrize("period", [PER, None])
+def test_rp_pi_tpcf_exact(mo, est, period):
+    s1, s2, r = _samples()
+    rp = np.linspace(0.01, 0.2, 6)
+    pi = np.linspace(0.0, 0.3, 5)
+    got = mo.rp_pi_tpcf(s1, rp, pi, randoms=r, period=period, estimator=est)
+    assert got.shape == (5, 4)
+    assert np.array_equal(got, OC.rp_pi_tpcf(s1, rp, pi, randoms=r, period=period, estimator_name=est))
+
+
+def test_rp_pi_tpcf_cross_and_analytic(mo):
+    # TPC/tests/test_rp_pi_tpcf.py:50-123
+    s1, s2, r = _samples()
+    rp = np.linspace(0.01, 0.2, 6)
+    pi = np.linspace(0.0, 0.3, 5)
+    got = mo.rp_pi_tpcf(s1, rp, pi, sample2=s2, period=1.0)
+    ref = OC.rp_pi_tpcf(s1, rp, pi, sample2=s2, period=PER)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+    cross = mo.rp_pi_tpcf(s1, rp, pi, sample2=s2, randoms=r, period=None, do_auto=False)
+    assert np.array_equal(cross, OC.rp_pi_tpcf(s1, rp, pi, sample2=s2, randoms=r, period=None, do_auto=False))
+
+
+def test_wp_exact(mo):
+    # TPC/tests/test_wp.py:24-150
+    s1, s2, r = _samples()
+    rp = np.logspace(-2, np.log10(0.2), 8)
+    assert np.array_equal(mo.wp(s1, rp, 0.25, period=1.0), OC.wp(s1, rp, 0.25, period=PER))
+    assert np.array_equal(mo.wp(s1, rp, 0.25, randoms=r), OC.wp(s1, rp, 0.25, randoms=r))
+    got = mo.wp(s1, rp, 0.25, sample2=s2, period=1.0)
+    assert len(got) == 3
+    for a, b in zip(got, OC.wp(s1, rp, 0.25, sample2=s2, period=PER)):
+        assert np.array_equal(a, b)
+    auto1, auto2 = mo.wp(s1, rp, 0.25, sample2=s2, period=1.0, do_cross=False)
+    assert np.array_equal(auto1, got[0]) and np.array_equal(auto2, got[2])
+    assert np.array_equal(mo.wp(s1, rp, 0.25, sample2=s2, period=1.0, do_auto=False), got[1])
+
+
+@pytest.mark.parametrize("normalize_by", ["number_counts", "random_marks"])
+@pytest.mark.parametrize("period", [1.0, None])
+def test_marked_tpcf_vs_oracle(mo, normalize_by, period):
+    s1 = random_sample(2000, seed=fixed_seed)
+    w1 = np.random.RandomState(fixed_seed).random_sample(2000)
+    rb = np.linspace(0.01, 0.25, 6)
+    got = mo.marked_tpcf(s1, rb, marks1=w1, period=period, normalize_by=normalize_by, seed=fixed_seed)
+    ref = OC.marked_tpcf(s1, rb, marks1=w1, period=None if period is None else PER, normalize_by=normalize_by,
+                         seed=fixed_seed)
+    np.testing.assert_allclose(got, ref, rtol=1e-12)
+
+
+def test_marked_tpcf_cross_iterations_and_consistency(mo):
+    # TPC/tests/test_marked_tpcf.py:67-140
+    s1, s2 = random_sample(1000, seed=1), random_sample(900, seed=2)
+    w1 = np.random.RandomState(3).random_sample((1000, 2))
+    w2 = np.random.RandomState(4).random_sample((900, 2))
+    rb = np.linspace(0.01, 0.25, 5)
+    for wid in (1, 3):
+        got = mo.marked_tpcf(s1, rb, sample2=s2, marks1=w1[:, :1] if wid == 1 else w1,
+                             marks2=w2[:, :1] if wid == 1 else w2, period=1, weight_func_id=wid, seed=7,
+                             iterations=3, randomize_marks=None if wid == 1 else [False, True])
+        ref = OC.marked_tpcf(s1, rb, sample2=s2, marks1=w1[:, :1] if wid == 1 else w1,
+                             marks2=w2[:, :1] if wid == 1 else w2, period=PER, weight_func_id=wid, seed=7,
+                             iterations=3, randomize_marks=None if wid == 1 else [False, True])
+        for a, b in zip(got, ref):
+            np.testing.assert_allclose(a, b, rtol=1e-12)
+    c1 = mo.marked_tpcf(s1, rb, sample2=s2, marks1=w1[:, 0], marks2=w2[:, 0], period=1, do_auto=False,
+                        normalize_by="number_counts")
+    a1, c2, a2 = mo.marked_tpcf(s1, rb, sample2=s2, marks1=w1[:, 0], marks2=w2[:, 0], period=1,
+                                normalize_by="number_counts")
+    assert np.array_equal(c1, c2)
+
+
+def test_caller_inputs_unchanged_and_device_tensors(mo, gpu):
+    import torch
+    s1, s2, r = _samples()
+    w = np.random.RandomState(5).random_sample(1000)
+    w_copy = w.copy()
+    rb = np.linspace(0.01, 0.25, 6)
+    host = mo.marked_tpcf(s1, rb, marks1=w, period=1.0, seed=3)
+    assert np.array_equal(w, w_copy)  # the in-place shuffle works on a copy, as in the reference
+    t1 = torch.tensor(s1, device=gpu)
+    tw = torch.tensor(w, device=gpu)
+    dev = mo.marked_tpcf(t1, rb, marks1=tw, period=1.0, seed=3)
+    assert isinstance(dev, torch.Tensor) and dev.is_cuda
+    assert torch.equal(tw, torch.tensor(w_copy, device=gpu))
+    np.testing.assert_allclose(dev.cpu().numpy(), host, rtol=1e-12)
+    xi_h = mo.tpcf(s1, rb, randoms=r, period=1.0, estimator="Landy-Szalay")
+    xi_d = mo.tpcf(t1, rb, randoms=torch.tensor(r, device=gpu), period=1.0, estimator="Landy-Szalay")
+    assert isinstance(xi_d, torch.Tensor) and np.array_equal(xi_d.cpu().numpy(), xi_h)
+    wp_d = mo.wp(t1, rb, 0.2, period=1.0)
+    assert isinstance(wp_d, torch.Tensor) and np.array_equal(wp_d.cpu().numpy(), mo.wp(s1, rb, 0.2, period=1.0))
+
+
+def test_tpcf_larger_box(mo):
+    # 200k points in L = 250 with 15 log bins to 20: the DD counts at a realistic size
+    rng = np.random.default_rng(11)
+    s = rng.uniform(0, 250.0, (200000, 3))
+    rb = np.logspace(-1, np.log10(20.0), 15)
+    got = mo.tpcf(s, rb, period=250.0)
+    ref = OC.tpcf(s, rb, period=np.array([250.0] * 3), num_threads=8)
+    assert np.array_equal(got, ref)
