@@ -77,7 +77,8 @@ def launch(res, stream=None):
 
 def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0, wfunc=0,
             want_counts=True, want_sums=False, rows1=False, rows2=False, want_evals=False,
-            also_counts=False, device=None, pair_events=None, build_events=None, tile_waves=0, tangent=False):
+            also_counts=False, device=None, pair_events=None, build_events=None, tile_waves=0, tangent=False,
+            jtags1=None, jtags2=None, n_samples=0):
     """Build a request with all of its output and scratch buffers allocated.
 
     mocks : list of dicts with n1, n2, period (3,), pbc, cell (3,) (cell <= 0: default)
@@ -87,6 +88,9 @@ def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0
         around the pair kernel(s) and around the cell build
     tile_waves : wavefronts per pair-kernel workgroup (1, 2, 4, 8; 0 = library default)
     tangent : marked id 1 with (w, dw/dtheta) weights: also res.tangents, d(sums)/dtheta
+    jtags1, jtags2, n_samples : jackknife kinds: int32 device tags (1..n_samples);
+        w1 / w2 are then optional (n,) weights and res.sums has shape
+        (M, n_samples + 1, nr) or (M, n_samples + 1, nr, np)
     The returned object keeps every buffer the request points to alive; the
     same request can be launched again (e.g. inside HIP-graph capture) after
     its input buffers are refreshed in place.
@@ -142,7 +146,7 @@ def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0
     res = Result()
     obj = kind == E.HP_NPAIRS_PER_OBJECT_3D
     ncount = nr * (len(p2) if kind == E.HP_NPAIRS_XY_Z else 1)
-    need_counts = (not marked) or also_counts
+    need_counts = (not marked and kind not in (E.HP_NPAIRS_JACKKNIFE_3D, E.HP_NPAIRS_JACKKNIFE_XY_Z)) or also_counts
     # counts and the status word share one buffer so one copy brings both back
     clen = n1_tot * nr if obj else M * ncount
     cbuf = torch.empty(clen + 1 if need_counts else 1, dtype=torch.int64, device=device)
@@ -154,8 +158,15 @@ def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0
     else:
         res.counts = None
     req.status = res.status.data_ptr()
+    jack = kind in (E.HP_NPAIRS_JACKKNIFE_3D, E.HP_NPAIRS_JACKKNIFE_XY_Z)
     sum_shape = (M, nr, len(p2)) if kind == E.HP_MARKED_NPAIRS_XY_Z else (M, nr)
-    res.sums = torch.empty(sum_shape, dtype=torch.float64, device=device) if (marked and want_sums) else None
+    if jack:
+        sum_shape = (M, int(n_samples) + 1, nr) + ((len(p2),) if kind == E.HP_NPAIRS_JACKKNIFE_XY_Z else ())
+        req.jtags1 = jtags1.data_ptr()
+        req.jtags2 = jtags2.data_ptr() if (jtags2 is not None and not same) else None
+        req.n_samples = int(n_samples)
+    res.sums = (torch.empty(sum_shape, dtype=torch.float64, device=device)
+                if ((marked and want_sums) or jack) else None)
     req.sums = res.sums.data_ptr() if res.sums is not None else None
     res.tangents = torch.empty((M, nr), dtype=torch.float64, device=device) if (marked and tangent) else None
     req.tangents = res.tangents.data_ptr() if res.tangents is not None else None
@@ -181,11 +192,13 @@ def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0
     res.fn = {E.HP_NPAIRS_3D: lib.hp_npairs_3d, E.HP_NPAIRS_PROJECTED: lib.hp_npairs_projected,
               E.HP_NPAIRS_XY_Z: lib.hp_npairs_xy_z, E.HP_MARKED_NPAIRS_3D: lib.hp_marked_npairs_3d,
               E.HP_NPAIRS_PER_OBJECT_3D: lib.hp_npairs_per_object_3d,
-              E.HP_MARKED_NPAIRS_XY_Z: lib.hp_marked_npairs_xy_z}[kind]
+              E.HP_MARKED_NPAIRS_XY_Z: lib.hp_marked_npairs_xy_z,
+              E.HP_NPAIRS_JACKKNIFE_3D: lib.hp_npairs_jackknife_3d,
+              E.HP_NPAIRS_JACKKNIFE_XY_Z: lib.hp_npairs_jackknife_xy_z}[kind]
     res.req = req
     res.device = device
     # keep every host and device buffer the request points to alive with it
-    res.keep = (marr, r2, p2, s1, s2, w1, w2)
+    res.keep = (marr, r2, p2, s1, s2, w1, w2, jtags1, jtags2)
     res.workspace = ws
     res.cbuf = cbuf
     return res

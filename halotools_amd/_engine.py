@@ -19,6 +19,8 @@ HP_NPAIRS_XY_Z = 2
 HP_MARKED_NPAIRS_3D = 3
 HP_NPAIRS_PER_OBJECT_3D = 4
 HP_MARKED_NPAIRS_XY_Z = 5
+HP_NPAIRS_JACKKNIFE_3D = 6
+HP_NPAIRS_JACKKNIFE_XY_Z = 7
 
 HP_SAME_SAMPLES = 1
 HP_ROWS1 = 2
@@ -30,7 +32,7 @@ HP_TANGENT = 32
 EXPORTED_SYMBOLS = (
     "hp_last_error", "hp_abi_version", "hp_workspace_size", "hp_npairs_3d",
     "hp_npairs_projected", "hp_npairs_xy_z", "hp_marked_npairs_3d", "hp_npairs_per_object_3d",
-    "hp_marked_npairs_xy_z", "hp_marked_backward",
+    "hp_marked_npairs_xy_z", "hp_npairs_jackknife_3d", "hp_npairs_jackknife_xy_z", "hp_marked_backward",
     "hp_segment_sum_scratch", "hp_segment_sum", "hp_debug_bin_lookup", "hp_debug_hilbert",
 )
 
@@ -75,6 +77,10 @@ class HpRequest(ctypes.Structure):
         ("pair_events", ctypes.c_void_p * 2),
         ("build_events", ctypes.c_void_p * 2),
         ("tangents", ctypes.c_void_p),
+        ("jtags1", ctypes.c_void_p),
+        ("jtags2", ctypes.c_void_p),
+        ("n_samples", ctypes.c_int32),
+        ("reserved2", ctypes.c_int32),
     ]
 
 
@@ -100,7 +106,8 @@ def library():
     lib.hp_workspace_size.argtypes = [ctypes.POINTER(HpRequest)]
     lib.hp_workspace_size.restype = ctypes.c_size_t
     for name in ("hp_npairs_3d", "hp_npairs_projected", "hp_npairs_xy_z", "hp_marked_npairs_3d",
-                 "hp_npairs_per_object_3d", "hp_marked_npairs_xy_z"):
+                 "hp_npairs_per_object_3d", "hp_marked_npairs_xy_z", "hp_npairs_jackknife_3d",
+                 "hp_npairs_jackknife_xy_z"):
         fn = getattr(lib, name)
         fn.argtypes = [ctypes.POINTER(HpRequest)]
         fn.restype = ctypes.c_int

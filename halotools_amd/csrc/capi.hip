@@ -150,6 +150,10 @@ struct Plan {
     bool tangent = false;
     bool obj = false;  // per-object counts
     bool mxyz = false;  // marked_npairs_xy_z: 2-D weighted sums
+    bool jack = false;  // jackknife kinds: (w, tag) rows, subsample tables
+    int nsamp = 0, jslots = 0;
+    size_t o_pk[2] = {0, 0}, o_jacc = 0;
+    bool weighted() const { return marked || jack; }
     size_t o_ptan = 0;
     size_t total = 0;
     std::vector<unsigned char> consts;  // bytes [0, o_gm[0])
@@ -177,13 +181,20 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     P.same = (r->flags & HP_SAME_SAMPLES) != 0;
     P.mxyz = (kind == HP_MARKED_NPAIRS_XY_Z);
     P.marked = (kind == HP_MARKED_NPAIRS_3D) || P.mxyz;
+    P.jack = (kind == HP_NPAIRS_JACKKNIFE_3D || kind == HP_NPAIRS_JACKKNIFE_XY_Z);
     P.obj = (kind == HP_NPAIRS_PER_OBJECT_3D);
     P.rows1 = P.marked && (r->flags & HP_ROWS1);
     P.rows2 = P.marked && (r->flags & HP_ROWS2);
     P.sums = P.marked && !(r->flags & HP_NO_SUMS);
-    P.counts = (!P.marked && !P.obj) || (P.marked && (r->flags & HP_ALSO_COUNTS));
+    P.counts = (!P.marked && !P.obj && !P.jack) || (P.marked && (r->flags & HP_ALSO_COUNTS));
     P.tangent = P.marked && (r->flags & HP_TANGENT);
-    P.nw = P.marked ? r->nw : 0;
+    P.nw = P.marked ? r->nw : (P.jack ? 2 : 0);
+    if (P.jack) {
+        if (r->flags & ~HP_SAME_SAMPLES) return fail("jackknife counts take no flags besides HP_SAME_SAMPLES");
+        if (r->n_samples < 1) return fail("n_samples must be >= 1");
+        if ((r->w1 || r->w2) && r->nw != 1) return fail("jackknife weights are one per point (nw = 1)");
+        P.nsamp = r->n_samples;
+    }
     if (r->tile_waves != 0 && r->tile_waves != 1 && r->tile_waves != 2 && r->tile_waves != 4 && r->tile_waves != 8)
         return fail("tile_waves must be 0, 1, 2, 4 or 8");
     if (P.marked) {
@@ -200,7 +211,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     P.Kr = r->nr;
     if (P.Kr < 1 || P.Kr > HP_MAX_BINS) return fail("number of radial bins out of range");
     if (!r->r2) return fail("r2 must not be null");
-    const bool xyz = (kind == HP_NPAIRS_XY_Z) || P.mxyz;  // 2-D (rp, pi) histogram
+    const bool xyz = (kind == HP_NPAIRS_XY_Z) || P.mxyz || (kind == HP_NPAIRS_JACKKNIFE_XY_Z);  // 2-D histogram
     if (xyz || kind == HP_NPAIRS_PROJECTED) {
         P.Kp = r->np;
         if (P.Kp < 1 || P.Kp > HP_MAX_BINS || !r->p2) return fail("pi bins out of range");
@@ -355,7 +366,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
                 S.x = take(n * 8);
                 S.y = take(n * 8);
                 S.z = take(n * 8);
-                if (P.marked) S.w = take(n * 8 * (size_t)P.nw);
+                if (P.weighted()) S.w = take(n * 8 * (size_t)P.nw);
             }
             S.perm = take(n * 4);
             if (o == kRowMajor) S.cs = take((size_t)(P.total_cells + 1) * 4);
@@ -390,6 +401,18 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     // unweighted auto-counts evaluate each unshifted unordered pair once; the
     // Hilbert ranks reuse the sort scratch (free once both sides are built)
     P.sym = P.same && !P.obj && (!P.marked || P.tangent);
+    if (P.jack) {
+        // (w, tag) rows of each side, and the per-mock subsample tables
+        P.o_pk[0] = take((size_t)std::max<int64_t>(P.n[0], 1) * 16);
+        if (!P.same) P.o_pk[1] = take((size_t)std::max<int64_t>(P.n[1], 1) * 16);
+        const double tab = (double)P.M * (P.nsamp + 1.0) * (double)P.nb_count * 8.0;
+        if (tab > 64.0 * 1024 * 1024 * 1024) return fail("jackknife tables exceed 64 GiB");
+        P.o_jacc = take((size_t)tab);
+        // LDS rows per wave for the tags a group meets: ~24 KB of table per wave
+        const int jrow = (P.Kr + 1) * (xyz ? P.Kp + 1 : 1) * 8;
+        P.jslots = std::max(0, std::min({kJackSlots, P.nsamp, 24576 / jrow - 1}));
+        if (jrow > 48 * 1024) return fail("jackknife: (len(rp_bins)+1)*(len(pi_bins)+1) exceeds 6144 cells");
+    }
     P.total = align_up(off, 256);
 
     // ---- constant blob (geometry, edges, tables) that goes to the workspace
@@ -438,6 +461,9 @@ static int run(const hp_request* r, int kind) {
         return fail("sample2 coordinates must not be null");
     if (P.marked && ((P.n[0] > 0 && !r->w1) || (!P.same && P.n[1] > 0 && !r->w2)))
         return fail("weights must not be null");
+    if (P.jack && ((P.n[0] > 0 && !r->jtags1) || (!P.same && P.n[1] > 0 && !r->jtags2)))
+        return fail("jackknife tags must not be null");
+    if (P.jack && !r->sums) return fail("sums output must not be null");
     if ((P.counts || P.obj) && !r->counts) return fail("counts output must not be null");
     if (P.sums && !r->sums) return fail("sums output must not be null");
     if (P.tangent && !r->tangents) return fail("tangents output must not be null");
@@ -455,6 +481,15 @@ static int run(const hp_request* r, int kind) {
     const double* Y[2] = {r->y1, P.same ? r->y1 : r->y2};
     const double* Z[2] = {r->z1, P.same ? r->z1 : r->z2};
     const double* W[2] = {r->w1, P.same ? r->w1 : r->w2};
+    if (P.jack) {
+        double* pk0 = reinterpret_cast<double*>(ws + P.o_pk[0]);
+        double* pk1 = P.same ? pk0 : reinterpret_cast<double*>(ws + P.o_pk[1]);
+        if (launch_pack_jack(r->w1, r->jtags1, P.n[0], P.nsamp, pk0, status, s) ||
+            (!P.same && launch_pack_jack(r->w2, r->jtags2, P.n[1], P.nsamp, pk1, status, s)))
+            return fail("jackknife pack launch failed");
+        W[0] = pk0;
+        W[1] = pk1;
+    }
 
     SortBufs sb;
     sb.keys[0] = reinterpret_cast<uint32_t*>(ws + P.o_keys[0]);
@@ -470,7 +505,7 @@ static int run(const hp_request* r, int kind) {
             const Side& S = P.sorted[sd][o];
             if (!S.needed) continue;
             // with SAME_SAMPLES only side 0 exists; the geometry rows of side 0 describe it
-            if (build_sorted_side(dgeom, P.M, sd, (Order)o, X[sd], Y[sd], Z[sd], P.marked ? W[sd] : nullptr, P.nw,
+            if (build_sorted_side(dgeom, P.M, sd, (Order)o, X[sd], Y[sd], Z[sd], P.weighted() ? W[sd] : nullptr, P.nw,
                                   P.n[sd], P.total_cells, o == kRowMajor ? P.bits_row : P.bits_hilbert, sb,
                                   side_ptrs(ws, S), status, s))
                 return fail("cell build launch failed");
@@ -497,7 +532,7 @@ static int run(const hp_request* r, int kind) {
     L.geom = dgeom;
     L.group_mock = gm0;
     L.ngroups = P.ngroups[0];
-    L.lane = lane_ptrs(ws, P.sorted[0][kHilbert], X[0], Y[0], Z[0], P.marked ? W[0] : nullptr);
+    L.lane = lane_ptrs(ws, P.sorted[0][kHilbert], X[0], Y[0], Z[0], P.weighted() ? W[0] : nullptr);
     L.strm = side_ptrs(ws, P.sorted[s2][kRowMajor]);
     L.nw = P.nw;
     L.br = P.tr.spec;
@@ -519,6 +554,15 @@ static int run(const hp_request* r, int kind) {
             L.orows = r->counts;
             break;
         case HP_MARKED_NPAIRS_XY_Z: L.mode = kModeWXYZ; break;
+        case HP_NPAIRS_JACKKNIFE_3D:
+        case HP_NPAIRS_JACKKNIFE_XY_Z:
+            L.mode = (kind == HP_NPAIRS_JACKKNIFE_XY_Z) ? kModeJackXY : kModeJack;
+            L.jacc = reinterpret_cast<double*>(ws + P.o_jacc);
+            L.nsamp = P.nsamp;
+            L.jslots = P.jslots;
+            launch_fill_u32(reinterpret_cast<uint32_t*>(L.jacc), 0u, (int64_t)P.M * (P.nsamp + 1) * P.nb_count * 2,
+                            s);
+            break;
         default:
             L.mode = P.tangent ? kModeTan
                                : ((r->wfunc == 1 && P.nw == 1 && (P.rows1 || P.rows2)) ? kModeJac : kModeWSum);
@@ -574,6 +618,10 @@ static int run(const hp_request* r, int kind) {
         if (launch_prefix_counts(L.pcount, P.Kr, kind == HP_NPAIRS_XY_Z ? P.Kp : 0, P.M, P.crep, r->counts, s))
             return fail("reduction launch failed");
     }
+    if (P.jack) {
+        if (launch_jack_finish(L.jacc, P.Kr, kind == HP_NPAIRS_JACKKNIFE_XY_Z ? P.Kp : 0, P.M, P.nsamp, r->sums, s))
+            return fail("jackknife reduction launch failed");
+    }
     if (P.sums) {
         if (launch_reduce_sums_s(L.psum, P.Kr, P.mxyz ? P.Kp : 0, dgeom, P.M, 0, P.max_groups[0],
                                  reinterpret_cast<double*>(red), r->sums, s))
@@ -612,6 +660,8 @@ int hp_npairs_xy_z(const hp_request* req) { return run(req, HP_NPAIRS_XY_Z); }
 int hp_marked_npairs_3d(const hp_request* req) { return run(req, HP_MARKED_NPAIRS_3D); }
 int hp_npairs_per_object_3d(const hp_request* req) { return run(req, HP_NPAIRS_PER_OBJECT_3D); }
 int hp_marked_npairs_xy_z(const hp_request* req) { return run(req, HP_MARKED_NPAIRS_XY_Z); }
+int hp_npairs_jackknife_3d(const hp_request* req) { return run(req, HP_NPAIRS_JACKKNIFE_3D); }
+int hp_npairs_jackknife_xy_z(const hp_request* req) { return run(req, HP_NPAIRS_JACKKNIFE_XY_Z); }
 
 int hp_marked_backward(const double* rows, int32_t nr, const double* gsuffix, const int64_t* offsets,
                        int32_t nmocks, double* grad, void* stream) {

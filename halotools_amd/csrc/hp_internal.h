@@ -120,9 +120,16 @@ struct PairLaunch {
     // auto-counts (unweighted): Hilbert position of each stream point; each
     // unshifted unordered pair is then evaluated once and counted twice
     const uint32_t* hrank;
+    // jackknife modes: per-mock subsample tables [nmocks][nsamp + 1][nbins]
+    // (row 0: all pairs; row s: weight lost when subsample s is removed),
+    // accumulated with atomics; lane and stream weights are (w, tag) rows
+    double* jacc;
+    int nsamp;
+    int jslots;            // LDS rows per wave for subsample tags (<= kJackSlots)
 };
 enum Mode : int { kModeCount3D = 0, kModeProj = 1, kModeXYZ = 2, kModeWSum = 3, kModeJac = 4, kModeTan = 5,
-                  kModeObj = 6, kModeWXYZ = 7 };
+                  kModeObj = 6, kModeWXYZ = 7, kModeJack = 8, kModeJackXY = 9 };
+constexpr int kJackSlots = 16;  // most subsample tags a wave keeps in LDS
 int launch_pairs(const PairLaunch& L, hipStream_t s);
 
 // reduce.hip
@@ -133,6 +140,9 @@ int launch_segment_sum(const double* v, const int64_t* offsets, int nmocks, doub
 int launch_rows_dot(const double* rows, int nr, const double* gsuffix, const int64_t* offsets,
                     int nmocks, double* grad, hipStream_t s);
 int launch_add_rows(const double* a, const double* b, double* out, int64_t n, hipStream_t s);
+int launch_pack_jack(const double* w, const int32_t* tags, int64_t n, int nsamp, double* out, int32_t* status,
+                     hipStream_t s);
+int launch_jack_finish(double* acc, int nr, int np, int nmocks, int nsamp, double* out, hipStream_t s);
 
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ int64_t mock_of_point(const MockGeom* __restrict__ g, int nmocks, int side,

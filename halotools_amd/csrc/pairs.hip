@@ -167,6 +167,7 @@ struct PairArgs {
     int smem;           // dynamic LDS bytes
     int ring_off;       // byte offset of the survivor ring within a wave's LDS
     int tab_bytes;      // LDS bytes of the bucket tables, ahead of the wave areas
+    int jrow;           // jackknife: bytes of one subsample row of the wave's table
 };
 
 // Survivor ring per wave: up to kBatch-1 leftover entries plus one 64-point chunk.
@@ -178,16 +179,22 @@ static_assert(kRing >= kBatch - 1 + 64, "survivor ring too small for the batch s
 template <int MODE>
 constexpr int kWavesPerSimd =
     (MODE == kModeJac) ? 1
-                       : ((MODE == kModeWSum || MODE == kModeTan || MODE == kModeWXYZ) ? 4 : (MODE == kModeXYZ ? 5 : 6));
+                       : ((MODE == kModeWSum || MODE == kModeTan || MODE == kModeWXYZ || MODE == kModeJack ||
+                           MODE == kModeJackXY)
+                              ? 4
+                              : (MODE == kModeXYZ ? 5 : 6));
 // per-object counts: one u32 row set per lane (each lane owns its point's counts)
 
 template <int MODE, int SWAP, int ALSO, int MULTI, int WAVES>
 __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const PairArgs A) {
-    constexpr bool F64 = (MODE == kModeWSum || MODE == kModeJac || MODE == kModeTan || MODE == kModeWXYZ);
+    // jackknife: FP64 tables of (w1 w2) per bin, one row for all pairs and one
+    // per subsample tag the wave meets (weights are (w, tag) rows)
+    constexpr bool JK = (MODE == kModeJack || MODE == kModeJackXY);
+    constexpr bool F64 = (MODE == kModeWSum || MODE == kModeJac || MODE == kModeTan || MODE == kModeWXYZ || JK);
     // 2-D (rp, pi) binning: one histogram per wave, [Kr + 1][Kp + 1] cells
     // (u32 counts, or FP64 weighted sums), the last row / column being trash
-    constexpr bool XY = (MODE == kModeXYZ || MODE == kModeWXYZ);
-    constexpr uint32_t cell_bytes = (MODE == kModeWXYZ) ? 8u : 4u;
+    constexpr bool XY = (MODE == kModeXYZ || MODE == kModeWXYZ || MODE == kModeJackXY);
+    constexpr uint32_t cell_bytes = (MODE == kModeWXYZ || MODE == kModeJackXY) ? 8u : 4u;
     // LDS: the bucket tables (sized to this request's buckets), then one area per wave
     extern __shared__ __align__(16) unsigned char lds[];
     LdsEntry* const s_tr = reinterpret_cast<LdsEntry*>(lds);
@@ -210,10 +217,11 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     constexpr bool SHARE4 = (MODE == kModeWSum || MODE == kModeTan);
     constexpr bool SHARE2 = (MODE == kModeCount3D || MODE == kModeProj);
     const bool shared = (MODE == kModeXYZ) || (!F64 && A.shared_count);  // shared u32 counts
-    const uint32_t step_r = XY ? (uint32_t)(Kp + 1) * cell_bytes
-                               : (shared ? 4u : ((SHARE4 || SHARE2) ? 128u : (MODE == kModeObj ? 256u : 512u)));
+    const uint32_t step_r =
+        XY ? (uint32_t)(Kp + 1) * cell_bytes
+           : (JK ? 8u : (shared ? 4u : ((SHARE4 || SHARE2) ? 128u : (MODE == kModeObj ? 256u : 512u))));
     const uint32_t lane_off =
-        (shared || XY) ? 0u
+        (shared || XY || JK) ? 0u
                : (SHARE4 ? (uint32_t)(lane & 15) * 8u
                          : (SHARE2 ? (uint32_t)(lane & 31) * 4u : (uint32_t)lane * (MODE == kModeObj ? 4u : 8u)));
     const uint32_t cnt_col = SHARE4 ? (uint32_t)(lane & 31) * 4u : 0u;  // SHARE4 count rows
@@ -382,7 +390,84 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     // points are paired separately below; shifted images (whose two orders
     // may round apart) are evaluated from both sides, once each.
     const uint32_t* __restrict__ HR = L.hrank;
-    const bool sym = (!F64 || MODE == kModeTan) && !SWAP && HR != nullptr;
+    const bool sym = (!F64 || MODE == kModeTan || JK) && !SWAP && HR != nullptr;
+
+    // ---- jackknife tables.  Row 0 sums w1 w2 of every pair; the row of tag t
+    // sums what removing subsample t takes away from it: w1 w2 when both
+    // points carry t, half of it for each point that does when their tags
+    // differ.  Subsample s of the result is then row 0 minus the row of s.
+    // Tags get an LDS row in the order the wave meets them (the lane points'
+    // tags first); once the rows run out, a tag's share goes straight to the
+    // mock's table in HBM.
+    const int nbj = XY ? Kr * Kp : Kr;
+    int stag[kJackSlots];  // tag of LDS row r + 1 (wave-uniform)
+    int ns = 0;            // rows in use
+    int lslot = 0;         // this lane's row (0: none)
+    int ltag = 0;
+    if (JK) {
+#pragma unroll
+        for (int j = 0; j < kJackSlots; j++) stag[j] = 0;
+        ltag = (int)lw[1];
+        uint64_t pend = (uint64_t)__ballot(act && ltag >= 1 && ltag <= L.nsamp);
+        while (pend) {
+            const int t = __builtin_amdgcn_readlane(ltag, __builtin_ctzll(pend));
+            const uint64_t mm = (uint64_t)__ballot(act && ltag == t);
+            if (ns < L.jslots) {
+#pragma unroll
+                for (int j = 0; j < kJackSlots; j++)
+                    if (j == ns) stag[j] = t;
+                ns++;
+                if (act && ltag == t) lslot = ns;
+            }
+            pend &= ~mm;
+        }
+    }
+    int jlast_t = -1, jlast_s = 0;
+    // row of a (wave-uniform) tag, assigning a free row on first sight
+    auto slot_of = [&](int t) __attribute__((always_inline)) -> int {
+        if (t == jlast_t) return jlast_s;
+        int sl = 0;
+#pragma unroll
+        for (int j = 0; j < kJackSlots; j++)
+            if (j < ns && stag[j] == t) sl = j + 1;
+        if (sl == 0 && ns < L.jslots && t >= 1 && t <= L.nsamp) {
+#pragma unroll
+            for (int j = 0; j < kJackSlots; j++)
+                if (j == ns) stag[j] = t;
+            ns++;
+            sl = ns;
+        }
+        jlast_t = t;
+        jlast_s = sl;
+        return sl;
+    };
+    auto jk_global = [&](int tag, uint32_t a, double val) __attribute__((always_inline)) {
+        if (tag < 1 || tag > L.nsamp) return;
+        const uint32_t c = a >> 3;
+        const int64_t b = XY ? (int64_t)(c / (uint32_t)(Kp + 1)) * Kp + (c % (uint32_t)(Kp + 1)) : (int64_t)c;
+        __hip_atomic_fetch_add(L.jacc + ((int64_t)m * (L.nsamp + 1) + tag) * nbj + b, val, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    };
+    // one pair in cell a: v = w1 w2 (times 2 for an unshifted pair under symmetry)
+    auto jk_add = [&](uint32_t a, double v, int ts, int ss) __attribute__((always_inline)) {
+        __hip_atomic_fetch_add(reinterpret_cast<double*>(wsm + a), v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        const bool same_t = (ltag == ts);
+        const double lv = same_t ? v : 0.5 * v;
+        if (lslot > 0)
+            __hip_atomic_fetch_add(reinterpret_cast<double*>(wsm + lslot * A.jrow + a), lv, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            jk_global(ltag, a, lv);
+        if (!same_t) {
+            const double sv = 0.5 * v;
+            if (ss > 0)
+                __hip_atomic_fetch_add(reinterpret_cast<double*>(wsm + ss * A.jrow + a), sv, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            else
+                jk_global(ts, a, sv);
+        }
+    };
     const uint32_t hend = (uint32_t)(p0 + cnt);
     // r: mock-relative stream index
     // inc: count-mode increment (2 for an unshifted pair evaluated once for both orders)
@@ -479,24 +564,36 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                 }
             }
         }
-        if (MODE == kModeTan || MODE == kModeJac) {
+        if (MODE == kModeTan || MODE == kModeJac || JK) {
             // weights of the whole batch in one scalar-load wait; most
             // evaluated pairs lie beyond the largest edge, and their lanes skip
             // the FP64 atomics instead of adding to the trash row
             double bw[B], bt[B];
 #pragma unroll
             for (int u = 0; u < B; u++) {
-                if (MODE == kModeTan) {
+                if (MODE == kModeTan || JK) {
                     bw[u] = sload_f64(SWm, off[u] * 2u);
                     bt[u] = sload_f64(SWm, off[u] * 2u + 8u);
                 } else {
                     bw[u] = sload_f64(SWm, off[u]);
                 }
             }
+            int jt[B], js[B];
+            if (JK) {
+                // tags and rows of the batch's stream points (wave-uniform;
+                // a row is only taken when some lane pairs with the point)
+#pragma unroll
+                for (int u = 0; u < B; u++) {
+                    jt[u] = uni((int)bt[u]);
+                    js[u] = ((uint64_t)__ballot(a[u] != trash_r) != 0) ? slot_of(jt[u]) : 0;
+                }
+            }
 #pragma unroll
             for (int u = 0; u < B; u++) {
                 if (a[u] == trash_r) continue;
-                if (MODE == kModeTan) {
+                if (JK) {
+                    jk_add(a[u], rl0 * bw[u], jt[u], js[u]);
+                } else if (MODE == kModeTan) {
                     // pair terms w_i w_j and t_i w_j + w_i t_j (t = dw/dtheta)
                     const double v = rl0 * bw[u];
                     const double tv = rl1 * bw[u] + rl0 * bt[u];
@@ -600,6 +697,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                     rinc = (sym && img == 13) ? 2u : 1u;
                     rcorr = MODE == kModeJac && !SWAP && L.corr != nullptr && img != 13;
                     rhalf = MODE == kModeTan && sym && img != 13;
+                    if (JK) rl0 = (sym && img == 13) ? 2.0 * lw[0] : lw[0];
                     if (MODE == kModeTan) {
                         rl0 = rhalf ? 0.5 * lw[0] : lw[0];
                         rl1 = rhalf ? 0.5 * lw[1] : lw[1];
@@ -660,7 +758,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                 bx[u] = sload_f64_at(L.lane.x, qk);
                 by[u] = sload_f64_at(L.lane.y, qk);
                 bz[u] = sload_f64_at(L.lane.z, qk);
-                if (MODE == kModeTan) {
+                if (MODE == kModeTan || JK) {
                     bw[u] = sload_f64_at(L.lane.w, 2 * qk);
                     bt[u] = sload_f64_at(L.lane.w, 2 * qk + 1);
                 }
@@ -673,6 +771,15 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                 sep(bx[u], by[u], bz[u], 0.0, 0.0, 0.0, 0, 0, 0, px, py, pz, dx, dy, dz);
                 const int k = k0 + u;
                 a[u] = (k < cnt && k >= lane) ? addr_of(dx, dy, dz) : trash_r;
+            }
+            if (JK) {
+#pragma unroll
+                for (int u = 0; u < B; u++) {
+                    const int ts = uni((int)bt[u]);
+                    const int ss = ((uint64_t)__ballot(a[u] != trash_r) != 0) ? slot_of(ts) : 0;
+                    if (a[u] != trash_r) jk_add(a[u], (((k0 + u > lane) ? 2.0 : 1.0) * lw[0]) * bw[u], ts, ss);
+                }
+                continue;
             }
 #pragma unroll
             for (int u = 0; u < B; u++) {
@@ -735,6 +842,31 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
             row[s] = run;
         }
     }
+    if (JK) {
+        // the wave's rows into the mock's table (row 0, then each tag's row)
+        const int cells = XY ? (Kr + 1) * (Kp + 1) : Kr + 1;
+        for (int r = 0; r <= ns; r++) {
+            int tag = 0;
+#pragma unroll
+            for (int j = 0; j < kJackSlots; j++)
+                if (j == r - 1) tag = stag[j];
+            const double* row = reinterpret_cast<const double*>(wsm + r * A.jrow);
+            double* dst = L.jacc + ((int64_t)m * (L.nsamp + 1) + tag) * nbj;
+            for (int c = lane; c < cells; c += 64) {
+                const double val = row[c];
+                if (val == 0.0) continue;
+                int64_t b = c;
+                if (XY) {
+                    const int kr = c / (Kp + 1), kp = c % (Kp + 1);
+                    if (kr >= Kr || kp >= Kp) continue;
+                    b = (int64_t)kr * Kp + kp;
+                } else if (c >= Kr) {
+                    continue;
+                }
+                __hip_atomic_fetch_add(dst + b, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
     if (MODE == kModeTan) {
         // rows of w_i w_j and of t_i w_j + w_i t_j, doubled under symmetry
         const double* th = reinterpret_cast<const double*>(wsm + A.tan_off);
@@ -795,6 +927,7 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
     A.shared_count = 0;
     A.cnt_off = 0;
     A.tan_off = 0;
+    A.jrow = 0;
     int wave_bytes = 0;
     switch (L.mode) {
         case kModeCount3D:
@@ -804,6 +937,12 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
             break;
         case kModeXYZ: wave_bytes = (Kr + 1) * (Kp + 1) * 4; break;
         case kModeWXYZ: wave_bytes = (Kr + 1) * (Kp + 1) * 8; break;
+        case kModeJack:
+        case kModeJackXY:
+            // row 0 plus one row per subsample tag the wave keeps in LDS
+            A.jrow = (Kr + 1) * (L.mode == kModeJackXY ? Kp + 1 : 1) * 8;
+            wave_bytes = (L.jslots + 1) * A.jrow;
+            break;
         case kModeObj: wave_bytes = (Kr + 1) * 64 * 4; break;
         case kModeJac:
             A.cnt_off = (Kr + 1) * 64 * 8;
@@ -823,7 +962,7 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
     if (L.br.nb > kTableMax || L.bp.nb > kTableMax) return -1;
     A.ring_off = align16(wave_bytes);
     A.wave_bytes = align16(A.ring_off + kRing * 4);
-    const bool xy = (L.mode == kModeXYZ || L.mode == kModeWXYZ);
+    const bool xy = (L.mode == kModeXYZ || L.mode == kModeWXYZ || L.mode == kModeJackXY);
     A.tab_bytes = align16((int)sizeof(LdsEntry) * (L.br.nb + (xy ? L.bp.nb : 0)));
     // Workgroup shape: the number of waves per workgroup that keeps the most
     // waves resident per CU (160 KB of LDS; register-limited waves per CU as
@@ -842,7 +981,8 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
     const int kLdsPerBlock = lds_per_block;
     const int reg_waves =
         4 * (L.mode == kModeJac ? 6
-                                : ((L.mode == kModeWSum || L.mode == kModeTan || L.mode == kModeWXYZ) ? 4
+                                : ((L.mode == kModeWSum || L.mode == kModeTan || L.mode == kModeWXYZ ||
+                                    L.mode == kModeJack || L.mode == kModeJackXY) ? 4
                                                                                                       : (L.mode == kModeXYZ ? 5 : 6)));
     int waves = 0, best = -1;
     if (L.waves > 0) {
@@ -866,6 +1006,8 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
         case kModeXYZ: return launch_m<kModeXYZ, 0, 0>(A, waves, multi, s);
         case kModeObj: return launch_m<kModeObj, 0, 0>(A, waves, multi, s);
         case kModeWXYZ: return launch_m<kModeWXYZ, 0, 0>(A, waves, multi, s);
+        case kModeJack: return launch_m<kModeJack, 0, 0>(A, waves, multi, s);
+        case kModeJackXY: return launch_m<kModeJackXY, 0, 0>(A, waves, multi, s);
         case kModeWSum:
             return also ? launch_m<kModeWSum, 0, 1>(A, waves, multi, s) : launch_m<kModeWSum, 0, 0>(A, waves, multi, s);
         case kModeJac:

@@ -298,4 +298,73 @@ int launch_add_rows(const double* a, const double* b, double* out, int64_t n, hi
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// ------------------------------------------------------------ jackknife
+// (w, tag) rows for the pair kernel: the weight (1 when none is given) and the
+// subsample tag as a double (exact below 2^53).  Tags outside [1, nsamp] set
+// status bit 2; the pair kernel never indexes a table with them.
+__global__ void k_pack_jack(const double* __restrict__ w, const int32_t* __restrict__ tags, int64_t n, int nsamp,
+                            double* __restrict__ out, int32_t* __restrict__ status) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t t = tags[i];
+        if ((t < 1 || t > nsamp) && status) atomicOr(status, 2);
+        out[2 * i] = w ? w[i] : 1.0;
+        out[2 * i + 1] = (double)t;
+    }
+}
+
+int launch_pack_jack(const double* w, const int32_t* tags, int64_t n, int nsamp, double* out, int32_t* status,
+                     hipStream_t s) {
+    if (n <= 0) return 0;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_pack_jack, dim3((unsigned)blocks), dim3(256), 0, s, w, tags, n, nsamp, out, status);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Inclusive prefix of every (mock, subsample) table row: over bins (1-D) or
+// over pi then rp (2-D, the order of k_reduce_s2).
+__global__ void __launch_bounds__(256) k_jack_prefix(double* __restrict__ acc, int nr, int np) {
+    double* o = acc + (int64_t)blockIdx.x * (np > 0 ? nr * np : nr);
+    if (np <= 0) {
+        if (threadIdx.x == 0) {
+            double run = 0.0;
+            for (int k = 0; k < nr; k++) { run += o[k]; o[k] = run; }
+        }
+        return;
+    }
+    for (int k = threadIdx.x; k < nr; k += blockDim.x) {
+        double run = 0.0;
+        for (int q = 0; q < np; q++) { run += o[k * np + q]; o[k * np + q] = run; }
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int q = threadIdx.x; q < np; q += blockDim.x) {
+        double run = 0.0;
+        for (int k = 0; k < nr; k++) { run += o[k * np + q]; o[k * np + q] = run; }
+    }
+}
+
+// out[m][0] = all pairs; out[m][s] = all pairs - weight lost without subsample s.
+__global__ void k_jack_out(const double* __restrict__ acc, int64_t nb, int nsamp, int64_t total,
+                           double* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = i % nb;
+        const int64_t row = i / nb;
+        const int64_t sub = row % (nsamp + 1);
+        const double h = acc[(row - sub) * nb + b];
+        out[i] = sub == 0 ? h : h - acc[i];
+    }
+}
+
+int launch_jack_finish(double* acc, int nr, int np, int nmocks, int nsamp, double* out, hipStream_t s) {
+    const int64_t rows = (int64_t)nmocks * (nsamp + 1);
+    const int64_t nb = np > 0 ? (int64_t)nr * np : nr;
+    hipLaunchKernelGGL(k_jack_prefix, dim3((unsigned)rows), dim3(np > 0 ? 256 : 64), 0, s, acc, nr, np);
+    const int64_t total = rows * nb;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_jack_out, dim3((unsigned)blocks), dim3(256), 0, s, acc, nb, nsamp, total, out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 }  // namespace hp

@@ -3,7 +3,8 @@
  *
  * Drop-in replacement for the native engines behind
  * halotools.mock_observables.{npairs_3d, npairs_projected, npairs_xy_z,
- * marked_npairs_3d, marked_npairs_xy_z, npairs_per_object_3d}.  The reference binds its engines as Cython `def`
+ * marked_npairs_3d, marked_npairs_xy_z, npairs_per_object_3d,
+ * npairs_jackknife_3d, npairs_jackknife_xy_z}.  The reference binds its engines as Cython `def`
  * functions that take a RectangularDoubleMesh plus the six coordinate arrays
  * and return a fresh NumPy array; here the mesh is built on the GPU and every
  * buffer is caller-owned device memory passed as a plain pointer.
@@ -50,7 +51,9 @@ enum hp_kind {
     HP_NPAIRS_XY_Z = 2,      /* PC/cpairs/npairs_xy_z_engine.pyx:17          */
     HP_MARKED_NPAIRS_3D = 3, /* PC/marked_cpairs/marked_npairs_3d_engine.pyx:22 */
     HP_NPAIRS_PER_OBJECT_3D = 4, /* PC/cpairs/npairs_per_object_3d_engine.pyx:19 */
-    HP_MARKED_NPAIRS_XY_Z = 5 /* PC/marked_cpairs/marked_npairs_xy_z_engine.pyx:22 */
+    HP_MARKED_NPAIRS_XY_Z = 5, /* PC/marked_cpairs/marked_npairs_xy_z_engine.pyx:22 */
+    HP_NPAIRS_JACKKNIFE_3D = 6, /* PC/cpairs/npairs_jackknife_3d_engine.pyx:17 */
+    HP_NPAIRS_JACKKNIFE_XY_Z = 7 /* PC/cpairs/npairs_jackknife_xy_z_engine.pyx:17 */
 };
 
 /* Bit flags of hp_request.flags */
@@ -98,7 +101,8 @@ typedef struct hp_request {
     /* outputs, device */
     int64_t* counts;           /* [nmocks][nr] cumulative (xy_z: [nmocks][nr][np])      */
     double* sums;              /* marked: [nmocks][nr] cumulative weighted sums         */
-                               /* (marked xy_z: [nmocks][nr][np])                      */
+                               /* (marked xy_z: [nmocks][nr][np]; jackknife:           */
+                               /* [nmocks][n_samples + 1][nr] or [..][nr][np])         */
     double* rows1;             /* HP_ROWS1: [n1_total][nr] per-bin (not cumulative)     */
     double* rows2;             /* HP_ROWS2: [n2_total][nr]                              */
     int64_t* evals;            /* optional [nmocks]: candidate pairs evaluated          */
@@ -111,6 +115,12 @@ typedef struct hp_request {
     void* build_events[2];     /* optional hipEvent_t pair recorded right before and   */
                                /* after the cell build (profiling)                     */
     double* tangents;          /* HP_TANGENT: [nmocks][nr] cumulative d(sums)/dtheta    */
+    /* jackknife kinds: subsample tag (1..n_samples) of every point, device int32;  */
+    /* w1 / w2 are then optional per-point weights (nw = 1), NULL meaning 1       */
+    const int32_t* jtags1;
+    const int32_t* jtags2;     /* ignored with HP_SAME_SAMPLES                        */
+    int32_t n_samples;
+    int32_t reserved2;
 } hp_request;
 
 /* Thread-local message describing the last error (empty string if none). */
@@ -156,6 +166,19 @@ int hp_marked_npairs_3d(const hp_request* req);
  * functions 1-15 it dispatches to (:231-263).  No flags beyond
  * HP_SAME_SAMPLES. */
 int hp_marked_npairs_xy_z(const hp_request* req);
+
+/* npairs_jackknife_3d: sums[m][s][k] = sum over pairs within r2[k] of the
+ * reference's jackknife weight of subsample s (0: w1*w2 for every pair; s >= 1:
+ * w1*w2 when neither point carries tag s, w1*w2/2 when exactly one does, 0
+ * when both do).  Replaces PC/cpairs/npairs_jackknife_3d_engine.pyx:17-236
+ * (jweight :239-287).  npairs_jackknife_xy_z: the same over (rp, pi) cells,
+ * sums[m][s][k][g]; replaces PC/cpairs/npairs_jackknife_xy_z_engine.pyx:17-246.
+ * Sums of unit weights (the default) are exact; with other weights each
+ * subsample row is row 0 minus the weight that subsample removes, summed with
+ * atomics (last-bit order effects).  A tag outside [1, n_samples] sets bit 2
+ * of the status word. */
+int hp_npairs_jackknife_3d(const hp_request* req);
+int hp_npairs_jackknife_xy_z(const hp_request* req);
 
 /* Backward of the product-weighted marked sums (no counterpart in the
  * reference): grad[p] = sum_s rows[p][s] * gsuffix[mock(p)][s], where

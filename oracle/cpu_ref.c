@@ -127,7 +127,8 @@ double ref_wfunc(int id, const double* w1, const double* w2) {
 /* The visitor is invoked for every (cell1, cell2) pair with the cell-2     */
 /* shift values, exactly where the Cython engines enter their i/j loops.    */
 
-enum { KIND_3D = 0, KIND_PROJ = 1, KIND_XYZ = 2, KIND_MARKED = 3, KIND_OBJ = 4, KIND_MARKED_XYZ = 5 };
+enum { KIND_3D = 0, KIND_PROJ = 1, KIND_XYZ = 2, KIND_MARKED = 3, KIND_OBJ = 4, KIND_MARKED_XYZ = 5,
+       KIND_JACK = 6, KIND_JACK_XYZ = 7 };
 
 typedef struct ref_job {
     int kind;
@@ -136,6 +137,8 @@ typedef struct ref_job {
     const double *w1, *w2;                     /* cell-sorted rows (n, nw) */
     int nw;
     int wfunc_id;
+    const int64_t *jt1, *jt2;                  /* jackknife: cell-sorted subvolume tags */
+    int64_t nsamp;                             /* jackknife: N_samples */
     const double* r2;   /* squared radial / rp bins (K)   */
     int K;
     const double* pi2;  /* squared pi bins (G) for xy_z   */
@@ -144,6 +147,17 @@ typedef struct ref_job {
     int64_t* counts;    /* int64 output (K) or (K*G)      */
     double* wcounts;    /* float64 output (K)             */
 } ref_job;
+
+/* jackknife weight of a pair for subsample s (0 = whole sample):
+ * PC/cpairs/npairs_jackknife_3d_engine.pyx:239-287 */
+static double jweight(int64_t s, int64_t j1, int64_t j2, double w1, double w2) {
+    double result = 0.0;
+    if (s == 0) result = w1 * w2;
+    else if ((j1 == j2) && (j1 == s)) result = 0.0;                  /* both outside */
+    else if ((j1 != s) && (j2 != s)) result = (w1 * w2);             /* both inside */
+    else if ((j1 != j2) && ((j1 == s) || (j2 == s))) result = 0.5 * (w1 * w2);
+    return result;
+}
 
 static inline int bin_pass(double dsq, double r2k) {
     return g_mutation == 1 ? (dsq < r2k) : (dsq <= r2k);
@@ -238,6 +252,38 @@ static void run_cells(const ref_job* J, int64_t first, int64_t last) {
                                     k = k - 1;
                                     if (k < 0) break;
                                 }
+                            } else if (J->kind == KIND_JACK) {
+                                /* PC/cpairs/npairs_jackknife_3d_engine.pyx:218-234 */
+                                const double dsq = dx * dx + dy * dy + dz * dz;
+                                const double w1 = J->w1[i], w2 = J->w2[j];
+                                const int64_t j1 = J->jt1[i], j2 = J->jt2[j];
+                                for (int64_t s = 0; s < J->nsamp + 1; s++) {
+                                    int k = K - 1;
+                                    while (bin_pass(dsq, J->r2[k])) {
+                                        J->wcounts[s * K + k] += jweight(s, j1, j2, w1, w2);
+                                        k = k - 1;
+                                        if (k < 0) break;
+                                    }
+                                }
+                            } else if (J->kind == KIND_JACK_XYZ) {
+                                /* PC/cpairs/npairs_jackknife_xy_z_engine.pyx:222-243 */
+                                const double dxy_sq = dx * dx + dy * dy;
+                                const double dz_sq = dz * dz;
+                                const double w1 = J->w1[i], w2 = J->w2[j];
+                                const int64_t j1 = J->jt1[i], j2 = J->jt2[j];
+                                for (int64_t s = 0; s < J->nsamp + 1; s++) {
+                                    int k = K - 1;
+                                    while (bin_pass(dxy_sq, J->r2[k])) {
+                                        int g = G - 1;
+                                        while (bin_pass(dz_sq, J->pi2[g])) {
+                                            J->wcounts[(s * K + k) * G + g] += jweight(s, j1, j2, w1, w2);
+                                            g = g - 1;
+                                            if (g < 0) break;
+                                        }
+                                        k = k - 1;
+                                        if (k < 0) break;
+                                    }
+                                }
                             } else if (J->kind == KIND_PROJ) {
                                 /* PC/cpairs/npairs_projected_engine.pyx:182-190 */
                                 const double dxy_sq = dx * dx + dy * dy;
@@ -292,8 +338,11 @@ static int run_threads(const ref_job* J, int nthreads) {
     const ref_mesh* m = J->m;
     const int64_t first = m->first_cell1, last = m->last_cell1;
     const int64_t ncells = last - first;
-    const int64_t outlen = (J->kind == KIND_XYZ || J->kind == KIND_MARKED_XYZ) ? (int64_t)J->K * J->G : J->K;
-    const int fp_out = (J->kind == KIND_MARKED || J->kind == KIND_MARKED_XYZ);
+    int64_t outlen = (J->kind == KIND_XYZ || J->kind == KIND_MARKED_XYZ) ? (int64_t)J->K * J->G : J->K;
+    if (J->kind == KIND_JACK) outlen = (J->nsamp + 1) * J->K;
+    if (J->kind == KIND_JACK_XYZ) outlen = (J->nsamp + 1) * J->K * J->G;
+    const int fp_out = (J->kind == KIND_MARKED || J->kind == KIND_MARKED_XYZ || J->kind == KIND_JACK ||
+                        J->kind == KIND_JACK_XYZ);
     /* per-object rows of different cell-1 chunks are disjoint: workers share
      * the output (the reference sums per-worker arrays that are zero elsewhere) */
     const int shared_out = (J->kind == KIND_OBJ);
@@ -444,6 +493,28 @@ int ref_marked_npairs_xy_z(const ref_mesh* m, const double* x1, const double* y1
     double* p2 = square_bins(pi_bins, G);
     J.r2 = r2; J.K = K; J.pi2 = p2; J.G = G; J.wcounts = wcounts;
     memset(wcounts, 0, sizeof(double) * (size_t)K * (size_t)G);
+    int rc = run_threads(&J, nthreads);
+    free(r2);
+    free(p2);
+    return rc;
+}
+
+/* npairs_jackknife_3d / _xy_z: wcounts is (N_samples+1, K) or (N_samples+1, K, G);
+ * weights and tags are cell-sorted (PC/cpairs/npairs_jackknife_3d_engine.pyx:76-84) */
+int ref_npairs_jackknife(const ref_mesh* m, const double* x1, const double* y1, const double* z1,
+                         const double* x2, const double* y2, const double* z2,
+                         const double* w1, const double* w2, const int64_t* jt1, const int64_t* jt2,
+                         int64_t nsamp, const double* rp_bins, int K, const double* pi_bins, int G,
+                         int nthreads, double* wcounts) {
+    ref_job J;
+    memset(&J, 0, sizeof(J));
+    J.kind = pi_bins ? KIND_JACK_XYZ : KIND_JACK; J.m = m;
+    J.x1 = x1; J.y1 = y1; J.z1 = z1; J.x2 = x2; J.y2 = y2; J.z2 = z2;
+    J.w1 = w1; J.w2 = w2; J.nw = 1; J.jt1 = jt1; J.jt2 = jt2; J.nsamp = nsamp;
+    double* r2 = square_bins(rp_bins, K);
+    double* p2 = pi_bins ? square_bins(pi_bins, G) : NULL;
+    J.r2 = r2; J.K = K; J.pi2 = p2; J.G = pi_bins ? G : 1; J.wcounts = wcounts;
+    memset(wcounts, 0, sizeof(double) * (size_t)(nsamp + 1) * (size_t)K * (size_t)J.G);
     int rc = run_threads(&J, nthreads);
     free(r2);
     free(p2);

@@ -77,6 +77,8 @@ def load(fast=False):
     lib.ref_marked_npairs_xy_z.argtypes = [ctypes.POINTER(_RefMeshC), P, P, P, P, P, P, P, P, I, I, P, I, P, I, I,
                                            P]
     lib.ref_npairs_per_object_3d.argtypes = [ctypes.POINTER(_RefMeshC), P, P, P, P, P, P, P, I, ctypes.c_int64, I, P]
+    lib.ref_npairs_jackknife.argtypes = [ctypes.POINTER(_RefMeshC), P, P, P, P, P, P, P, P, P, P, ctypes.c_int64, P,
+                                         I, P, I, I, P]
     lib.ref_candidate_pairs.argtypes = [ctypes.POINTER(_RefMeshC)]
     lib.ref_candidate_pairs.restype = ctypes.c_int64
     lib.ref_set_mutation.argtypes = [I]
@@ -389,6 +391,39 @@ def marked_npairs_xy_z(sample1, sample2, rp_bins, pi_bins, period=None, weights1
                                            pi_bins.ctypes.data, len(pi_bins), int(num_threads), out.ctypes.data)
     assert rc == 0
     return out
+
+
+def npairs_jackknife(sample1, sample2, rp_bins, jtags1, jtags2, N_samples, pi_bins=None, period=None,
+                     weights1=None, weights2=None, num_threads=1, approx_cell1_size=None, approx_cell2_size=None):
+    """Oracle for PC/npairs_jackknife_3d.py:24-95 (pi_bins None) and
+    PC/npairs_jackknife_xy_z.py:23-199: (N_samples+1, K) or (N_samples+1, K, G)
+    float64 (engine loops PC/cpairs/npairs_jackknife_3d_engine.pyx:98-236,
+    PC/cpairs/npairs_jackknife_xy_z_engine.pyx:98-246)."""
+    rp_bins = np.atleast_1d(rp_bins).astype("f8")
+    rmax = np.max(rp_bins)
+    zmax = rmax if pi_bins is None else np.max(pi_bins)
+    if pi_bins is not None:
+        pi_bins = np.atleast_1d(pi_bins).astype("f8")
+    n1, n2 = len(sample1), len(sample2)
+    w1 = np.ones(n1) if weights1 is None else np.asarray(weights1, dtype=np.float64)
+    w2 = np.ones(n2) if weights2 is None else np.asarray(weights2, dtype=np.float64)
+    jt1 = np.asarray(jtags1).astype(np.int64)
+    jt2 = np.asarray(jtags2).astype(np.int64)
+    mesh, cols = _prepare(sample1, sample2, (rmax, rmax, zmax), period,
+                          approx_cell1_size, approx_cell2_size, (rmax, rmax, zmax))
+    w1s = np.ascontiguousarray(w1[mesh.mesh1.idx_sorted])
+    w2s = np.ascontiguousarray(w2[mesh.mesh2.idx_sorted])
+    j1s = np.ascontiguousarray(jt1[mesh.mesh1.idx_sorted])
+    j2s = np.ascontiguousarray(jt2[mesh.mesh2.idx_sorted])
+    G = 1 if pi_bins is None else len(pi_bins)
+    out = np.zeros((N_samples + 1, len(rp_bins), G), dtype=np.float64)
+    mc = mesh.as_c()
+    rc = load().ref_npairs_jackknife(ctypes.byref(mc), *_ptrs(cols), w1s.ctypes.data, w2s.ctypes.data,
+                                     j1s.ctypes.data, j2s.ctypes.data, int(N_samples), rp_bins.ctypes.data,
+                                     len(rp_bins), None if pi_bins is None else pi_bins.ctypes.data, G,
+                                     int(num_threads), out.ctypes.data)
+    assert rc == 0
+    return out[:, :, 0] if pi_bins is None else out
 
 
 def candidate_pairs_3d(sample1, sample2, rmax, period, approx_cell1_size=None, approx_cell2_size=None):

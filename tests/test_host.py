@@ -44,7 +44,7 @@ def test_abi_struct_sizes():
     # hp_mock and hp_request must match the C layout of include/halopair.h
     assert ctypes.sizeof(E.HpMock) == 8 + 8 + 24 + 4 + 4 + 24
     off = E.HpRequest.workspace.offset
-    assert off % 8 == 0 and ctypes.sizeof(E.HpRequest) == E.HpRequest.tangents.offset + 8
+    assert off % 8 == 0 and ctypes.sizeof(E.HpRequest) == E.HpRequest.reserved2.offset + 4
 
 
 def test_abi_layout_matches_c_compiler(tmp_path):

@@ -1,0 +1,111 @@
+"""CPU tests of the jackknife pair counts (no GPU needed).
+
+The oracle (cpu_ref.c restating PC/cpairs/npairs_jackknife_3d_engine.pyx and
+npairs_jackknife_xy_z_engine.pyx) is pinned by an identity that uses only the
+already pinned plain and marked counters: removing subsample s keeps the pairs
+with both points outside s at full weight and the pairs with one point inside
+at half weight, so row s = W(out, out) + (W(out, in) + W(in, out)) / 2.
+The weights / tags checks follow PC/test_pair_counters/test_npairs_jackknife_3d.py:119-260.
+"""
+import numpy as np
+import pytest
+
+from halotools_amd.custom_exceptions import HalotoolsError
+from halotools_amd.mock_observables.pair_counters.npairs_jackknife_3d import (
+    _npairs_jackknife_3d_process_weights_jtags, npairs_jackknife_3d, npairs_jackknife_xy_z)
+from oracle import mesh as ORC
+from tests.helpers import fixed_seed, random_sample
+
+
+def _tags(s, n=2):
+    # n x n x n spatial subvolumes of the unit box, tags 1..n^3
+    i = np.minimum((s * n).astype(int), n - 1)
+    return 1 + (i[:, 0] * n + i[:, 1]) * n + i[:, 2]
+
+
+@pytest.mark.parametrize("period", [1.0, None])
+def test_oracle_jackknife_rows_identity(period):
+    s1, s2 = random_sample(400, seed=1), random_sample(350, seed=2)
+    t1, t2 = _tags(s1), _tags(s2)
+    w1 = np.random.RandomState(3).random_sample(400)
+    w2 = np.random.RandomState(4).random_sample(350)
+    rb = np.array([0.05, 0.1, 0.2, 0.3])
+    out = ORC.npairs_jackknife(s1, s2, rb, t1, t2, 8, period=period, weights1=w1, weights2=w2)
+    W = lambda a, b, wa, wb: ORC.marked_npairs_3d(a, b, rb, period=period, weights1=wa, weights2=wb,
+                                                  weight_func_id=1) if len(a) and len(b) else np.zeros(len(rb))
+    np.testing.assert_allclose(out[0], W(s1, s2, w1, w2), rtol=1e-13)
+    for k in range(1, 9):
+        o1, o2 = t1 != k, t2 != k
+        # the enclosing box of a subset differs for period=None: compare on the full sample's box
+        if period is None:
+            expect = None
+        else:
+            expect = (W(s1[o1], s2[o2], w1[o1], w2[o2]) +
+                      0.5 * (W(s1[o1], s2[~o2], w1[o1], w2[~o2]) + W(s1[~o1], s2[o2], w1[~o1], w2[o2])))
+        if expect is not None:
+            np.testing.assert_allclose(out[k], expect, rtol=1e-12)
+    # unit weights: exact integers / halves, equal to the plain counts
+    u = ORC.npairs_jackknife(s1, s2, rb, t1, t2, 8, period=period)
+    assert np.array_equal(u[0], ORC.npairs_3d(s1, s2, rb, period=period).astype(float))
+    if period is not None:
+        for k in range(1, 9):
+            o1, o2 = t1 != k, t2 != k
+            e = (ORC.npairs_3d(s1[o1], s2[o2], rb, period=period) +
+                 0.5 * (ORC.npairs_3d(s1[o1], s2[~o2], rb, period=period) +
+                        ORC.npairs_3d(s1[~o1], s2[o2], rb, period=period)))
+            assert np.array_equal(u[k], e)
+
+
+def test_oracle_jackknife_xy_z_identity():
+    s = random_sample(500, seed=5)
+    t = _tags(s)
+    rp, pi = np.array([0.05, 0.1, 0.2]), np.array([0.0, 0.1, 0.25])
+    out = ORC.npairs_jackknife(s, s, rp, t, t, 8, pi_bins=pi, period=1.0)
+    assert out.shape == (9, 3, 3)
+    assert np.array_equal(out[0], ORC.npairs_xy_z(s, s, rp, pi, period=1.0).astype(float))
+    for k in range(1, 9):
+        o = t != k
+        e = (ORC.npairs_xy_z(s[o], s[o], rp, pi, period=1.0) +
+             0.5 * (ORC.npairs_xy_z(s[o], s[~o], rp, pi, period=1.0) +
+                    ORC.npairs_xy_z(s[~o], s[o], rp, pi, period=1.0)))
+        assert np.array_equal(out[k], e)
+
+
+def _w_t(npts1=10, npts2=10, n=5):
+    rng = np.random.RandomState(fixed_seed)
+    return (rng.random_sample((npts1, 3)), rng.random_sample((npts2, 3)), rng.rand(npts1), rng.rand(npts2),
+            rng.randint(1, n, npts1), rng.randint(1, n, npts2))
+
+
+def test_process_weights_and_tags_messages():
+    s1, s2, w1, w2, t1, t2 = _w_t()
+    _npairs_jackknife_3d_process_weights_jtags(s1, s2, w1, w2, t1, t2, 5)
+    cases = [
+        (dict(weights1=w1[:-1]), "weights1 should have same len as sample1"),
+        (dict(weights2=w2[:-1]), "weights2 should have same len as sample2"),
+        (dict(jtags1=t1[:-1]), "jtags1 should have same len as sample1"),
+        (dict(jtags2=t2[:-1]), "jtags2 should have same len as sample2"),
+        (dict(jtags1=np.zeros(10, dtype=int)), "jtags1 must be >= 1"),
+        (dict(jtags2=np.zeros(10, dtype=int)), "jtags2 must be >= 1"),
+        (dict(jtags1=np.full(10, 7)), "jtags1 must be <= N_samples"),
+        (dict(jtags2=np.full(10, 7)), "jtags2 must be <= N_samples"),
+    ]
+    for kw, msg in cases:
+        args = dict(weights1=w1, weights2=w2, jtags1=t1, jtags2=t2)
+        args.update(kw)
+        with pytest.raises(HalotoolsError) as err:
+            _npairs_jackknife_3d_process_weights_jtags(s1, s2, args["weights1"], args["weights2"], args["jtags1"],
+                                                        args["jtags2"], 5)
+        assert msg in err.value.args[0]
+    with pytest.warns(UserWarning, match="sample1 does not contain points in every jackknife sample"):
+        _npairs_jackknife_3d_process_weights_jtags(s1, s2, w1, w2, np.ones(10, dtype=int), t2, 5)
+
+
+def test_drop_in_argument_errors_before_any_gpu_work():
+    s1, s2, w1, w2, t1, t2 = _w_t(100, 100)
+    with pytest.raises(HalotoolsError):
+        npairs_jackknife_3d(s1, s2, np.array([0.1, 0.2]), t1[:-1], t2, 5, period=1.0)
+    with pytest.raises(ValueError):
+        npairs_jackknife_3d(s1, s2, np.array([0.1, 0.5]), t1, t2, 5, period=1.0)
+    with pytest.raises(ValueError):
+        npairs_jackknife_xy_z(s1, s2, np.array([0.1, 0.2]), np.array([0.1, 0.5]), t1, t2, 5, period=1.0)
