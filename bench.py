@@ -29,7 +29,7 @@ ALG_BUILD_BYTES = 76
 # HBM bytes per launch of the pair kernel on this workload: rocprofv3
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors being what the same
 # counters read for a plain 256 MiB copy (profiles/k_pairs_count3d_clustered_1M.md)
-PAIR_KERNEL_TRAFFIC_BYTES = 3.926e8
+PAIR_KERNEL_TRAFFIC_BYTES = 4.120e8
 
 
 def reference_candidates(pts, rmax, L):
@@ -146,12 +146,15 @@ def bench_other_configs(dev, steps):
     """GPU-only timings of the other hot-path rows on their full-size inputs.
 
     2M uniform points in L = 1000: npairs_xy_z with 16 log rp edges 0.1-30 and
-    pi edges 0..40 step 1, and npairs_projected with pi_max = 40.  1M uniform
-    halos in L = 1000 with log10 M ~ U[11, 15]: product-weighted marked sums
-    with Zheng07 central weights (logMmin = 12, sigma = 0.3), forward plus
-    backward to (logMmin, sigma).  Inputs are resident in HBM; each entry is the
-    mean wall time of whole calls and, for the projected pair kernels, their
-    own pair evaluations per second from HIP events.
+    pi edges 0..40 step 1, npairs_projected with pi_max = 40, and wp.  1M
+    uniform halos in L = 1000 with log10 M ~ U[11, 15]: product-weighted marked
+    sums with Zheng07 central weights (logMmin = 12, sigma = 0.3), forward plus
+    backward to (logMmin, sigma), and marked_npairs_xy_z of the same weights.
+    The headline's clustered box: npairs_per_object_3d (its 200k-point
+    version), npairs_jackknife_3d with 125 spatial subvolumes, and tpcf with
+    analytic randoms.  Inputs are resident in HBM; each entry is the mean wall
+    time of whole calls and, where given, the pair kernel's own evaluations per
+    second from HIP events.
     """
     import numpy as np
     import torch
@@ -259,6 +262,27 @@ def bench_other_configs(dev, steps):
                     "weight_func_id 1 with Zheng07 weights",
         "ms_per_call": s_mx * 1e3, "pair_kernel_ms": ks * 1e3, "kernel_evals": evals,
         "kernel_evals_per_s": evals / ks, "fp64_peak_frac": 8.0 * evals / ks / 1e12 / FP64_PEAK_TFLOPS}
+    del t4, cols4, w4, res, wx
+
+    # jackknife counts and the two-point caller layer on the headline's clustered box
+    from halotools_amd.mock_observables import npairs_jackknife_3d, tpcf, wp
+    p2c = mocks.config_points(2)[0]
+    t2c = torch.from_numpy(np.ascontiguousarray(p2c)).to(dev)
+    rb2 = mocks.config_rbins(2)
+    sub = np.minimum((p2c / 1000.0 * 5).astype(np.int64), 4)
+    tags = torch.from_numpy((1 + (sub[:, 0] * 5 + sub[:, 1]) * 5 + sub[:, 2]).astype(np.int32)).to(dev)
+    jk, s_jk = timed(lambda: npairs_jackknife_3d(t2c, t2c, rb2, tags, tags, 125, period=1000.0))
+    out["npairs_jackknife_3d"] = {
+        "workload": "%d-point NFW-clustered box (as the headline), L=1000, 20 log edges 0.1-50, "
+                    "125 spatial subvolumes, unit weights, table (126, 20)" % len(p2c),
+        "ms_per_call": s_jk * 1e3}
+    _, s_tp = timed(lambda: tpcf(t2c, rb2, period=1000.0))
+    out["tpcf"] = {"workload": "xi(r) of the headline box, Natural estimator, analytic randoms "
+                               "(DD on the device, estimator on the device)", "ms_per_call": s_tp * 1e3}
+    t3 = torch.from_numpy(np.ascontiguousarray(mocks.config_points(3)[0])).to(dev)
+    _, s_wp = timed(lambda: wp(t3, mocks.config_rbins(3), 40.0, period=1000.0))
+    out["wp"] = {"workload": "wp(rp) of 2M uniform points, L=1000, 16 log rp edges 0.1-30, pi_max 40, "
+                             "analytic randoms", "ms_per_call": s_wp * 1e3}
     return out
 
 

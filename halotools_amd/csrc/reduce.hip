@@ -84,14 +84,30 @@ __global__ void __launch_bounds__(256) k_reduce_s2(const T* __restrict__ s1, int
 // integer atomics); this turns it into the reference's cumulative counts.
 __global__ void __launch_bounds__(256) k_prefix_counts(const uint64_t* __restrict__ acc, int nr, int np, int crep,
                                                        int64_t* __restrict__ out) {
+    __shared__ uint64_t sh[256];
     const int m = blockIdx.x;
     const int nb = np > 0 ? nr * np : nr;
     const uint64_t* a = acc + (int64_t)m * crep * nb;
     int64_t* o = out + (int64_t)m * nb;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-        uint64_t t = 0;
-        for (int r = 0; r < crep; r++) t += a[(int64_t)r * nb + b];
-        o[b] = (int64_t)t;
+    // replica sums: threads split over (bin, replica stripe), then the stripes
+    // are added in LDS (integer sums: exact in any order)
+    const int nbt = nb < 256 ? nb : 256;
+    const int rstride = 256 / nbt;
+    for (int b0 = 0; b0 < nb; b0 += nbt) {
+        const int t = threadIdx.x;
+        const int b = b0 + t % nbt;
+        const int rr = t / nbt;
+        uint64_t acc_t = 0;
+        if (rr < rstride && b < nb)
+            for (int r = rr; r < crep; r += rstride) acc_t += a[(int64_t)r * nb + b];
+        sh[t] = acc_t;
+        __syncthreads();
+        if (t < nbt && b0 + t < nb) {
+            uint64_t tot = 0;
+            for (int q = 0; q < rstride; q++) tot += sh[q * nbt + t];
+            o[b0 + t] = (int64_t)tot;
+        }
+        __syncthreads();
     }
     __threadfence_block();
     __syncthreads();
