@@ -113,6 +113,7 @@ struct alignas(16) LdsEntry {
     uint32_t addr;
     uint32_t k;
 };
+static_assert(sizeof(LdsEntry) == 16, "bucket entries are read as one 16-byte LDS load");
 
 // Byte offset of the first bin a value d falls in (bin k iff d <= e[k]):
 // bucket from the high word of d, one exact FP64 comparison with the only
@@ -142,6 +143,15 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb) {
 #define HP_BATCH 8
 #endif
 constexpr int kBatch = HP_BATCH;  // stream points per scalar-load wait
+#ifndef HP_SPLIT
+#define HP_SPLIT 1
+#endif
+#ifndef HP_SPLIT_CHUNK
+#define HP_SPLIT_CHUNK 2
+#endif
+#ifndef HP_COUNT_WAVES
+#define HP_COUNT_WAVES 6  // waves per SIMD the count kernels' registers are held to
+#endif
 
 // Read-only global data seen through the constant address space: uniform
 // loads from it always become scalar loads (SGPR results, no VGPRs).
@@ -182,7 +192,7 @@ constexpr int kWavesPerSimd =
                        : ((MODE == kModeWSum || MODE == kModeTan || MODE == kModeWXYZ || MODE == kModeJack ||
                            MODE == kModeJackXY)
                               ? 4
-                              : (MODE == kModeXYZ ? 5 : 6));
+                              : (MODE == kModeXYZ ? 5 : HP_COUNT_WAVES));
 // per-object counts: one u32 row set per lane (each lane owns its point's counts)
 
 template <int MODE, int SWAP, int ALSO, int MULTI, int WAVES>
@@ -336,10 +346,11 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     uint64_t bound = 0;
 
     // Pairs are processed in batches: the stream points of a batch arrive with
-    // one scalar-load wait, all distances are formed, then all bucket-table
-    // reads are issued back to back (one LDS wait), then the fire-and-forget
-    // LDS atomics.  Source order keeps the table reads ahead of the atomics,
-    // which the compiler may not reorder because both touch LDS.
+    // one scalar-load wait; for each chunk of HP_SPLIT_CHUNK slots the
+    // distances are formed, then the chunk's bucket-table reads are issued
+    // back to back (one LDS wait), then the compares; the fire-and-forget LDS
+    // atomics of the batch follow.  (Chunks of 2 keep the count kernel within
+    // its register budget; 2 instead of 1 slot per wait: -4 % kernel time.)
     auto sep = [&](double qx, double qy, double qz, double shx, double shy, double shz, int sxs, int sys, int szs,
                    double ax, double ay, double az, double& dx, double& dy, double& dz) __attribute__((always_inline)) {
         if (!SWAP) {
@@ -525,7 +536,43 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
         }
         evals += (uint64_t)cnt * (uint64_t)nv;
         uint32_t a[B];
-        if (nv == B) {
+        if (HP_SPLIT && !XY && !MULTI) {
+            // per chunk of C slots: the distances first, then all of their
+            // bucket reads, then the compares -- one LDS wait per chunk
+            // instead of one per slot (C bounds the live registers)
+            constexpr int C = (HP_SPLIT_CHUNK < B) ? HP_SPLIT_CHUNK : B;
+#pragma unroll
+            for (int h = 0; h < B; h += C) {
+                double dq[C], dz2[C];
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    const int u = h + c;
+                    double dx, dy, dz;
+                    sep(bx[u], by[u], bz[u], rshx, rshy, rshz, rsxs, rsys, rszs, rax, ray, raz, dx, dy, dz);
+                    if (MODE == kModeProj) {
+                        dq[c] = dx * dx + dy * dy;
+                        dz2[c] = dz * dz;
+                    } else {
+                        dq[c] = dx * dx + dy * dy + dz * dz;
+                    }
+                }
+                uint4 ent[C];
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    const uint32_t hi = (uint32_t)__double2hiint(dq[c]);
+                    const uint32_t b = min(__builtin_elementwise_sub_sat(hi, bshr) >> shr, nbr1);
+                    ent[c] = *reinterpret_cast<const uint4*>(s_tr + b);
+                }
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    const int u = h + c;
+                    const double edge = __hiloint2double((int)ent[c].y, (int)ent[c].x);
+                    uint32_t v = ent[c].z + (edge < dq[c] ? step_r : 0u);
+                    if (MODE == kModeProj) v = (dz2[c] <= pimax2) ? v : trash_r;
+                    a[u] = (u < nv) ? v : trash_r;
+                }
+            }
+        } else if (nv == B) {
 #pragma unroll
             for (int u = 0; u < B; u++) {
                 double dx, dy, dz;
@@ -983,7 +1030,7 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
         4 * (L.mode == kModeJac ? 6
                                 : ((L.mode == kModeWSum || L.mode == kModeTan || L.mode == kModeWXYZ ||
                                     L.mode == kModeJack || L.mode == kModeJackXY) ? 4
-                                                                                                      : (L.mode == kModeXYZ ? 5 : 6)));
+                                                                                                      : (L.mode == kModeXYZ ? 5 : HP_COUNT_WAVES)));
     int waves = 0, best = -1;
     if (L.waves > 0) {
         waves = L.waves;
