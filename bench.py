@@ -35,18 +35,19 @@ PAIR_KERNEL_TRAFFIC_BYTES = 4.120e8
 def reference_candidates(pts, rmax, L):
     """Candidate pair evaluations of the reference engine on an auto-count.
 
-    The reference mesh with its default cell sizes (approx_cell = rmax,
-    PC/npairs_3d.py:202-209) has n = max(3, min(floor(L/rmax), 50)) cells per
-    dimension for both samples and a 3x3x3 window of cell-2s around each cell-1
-    (PC/rectangular_mesh.py:25-82, PC/cpairs/npairs_3d_engine.pyx:74-123); it
-    evaluates n1(cell) * sum of n2 over the window for every cell.
+    The reference mesh with its default cell sizes (approx_cell = the search
+    length, PC/npairs_3d.py:202-209) has n_d = max(3, min(floor(L/search_d), 50))
+    cells per dimension for both samples and a 3x3x3 window of cell-2s around
+    each cell-1 (PC/rectangular_mesh.py:25-82, PC/cpairs/npairs_3d_engine.pyx:74-123);
+    it evaluates n1(cell) * sum of n2 over the window for every cell.  ``rmax``
+    is the search length, or a (x, y, z) triple of them ((rp, rp, pi) for the
+    projected counters).
     """
     import numpy as np
-    n = max(3, min(int(np.floor(L / rmax)), 50))
-    cs = L / n
-    idx = np.floor(pts // cs).astype(int)
-    idx = np.where(idx >= n, n - 1, idx)
-    h = np.zeros((n, n, n), dtype=np.int64)
+    search = np.broadcast_to(np.asarray(rmax, dtype=float), (3,))
+    n = [max(3, min(int(np.floor(L / s)), 50)) for s in search]
+    idx = np.stack([np.minimum(np.floor(pts[:, d] // (L / n[d])).astype(int), n[d] - 1) for d in range(3)], axis=1)
+    h = np.zeros(n, dtype=np.int64)
     np.add.at(h, (idx[:, 0], idx[:, 1], idx[:, 2]), 1)
     win = np.zeros_like(h)
     for dx in (-1, 0, 1):
@@ -62,9 +63,10 @@ def bench_batched(dev, steps):
     N_m log-uniform in [5e4, 2e6], uniform in a periodic L = 1000 box, log10 M
     uniform in [11, 15], Zheng07 central weights with logMmin_m ~ U[11.5, 12.5]
     and sigma = 0.3; loss_m = sum_k c_k W_mk.  The batched step (one HIP graph:
-    weights, one pair traversal for counts + Jacobian rows, backward
-    contraction, per-mock parameter gradients) is timed against the same work
-    done mock by mock through the drop-in API.
+    weights and their derivative, one forward-mode pair traversal for counts, W
+    and dW/dlogMmin, per-mock losses and gradients) is timed against the same
+    work done mock by mock through the drop-in API (counts, marked forward with
+    Jacobian rows, hand-written backward).
     """
     import numpy as np
     import torch
@@ -142,7 +144,27 @@ def bench_batched(dev, steps):
             "cell_build": _build_rate(npts, build_s)}
 
 
-def bench_other_configs(dev, steps):
+def _cpu_leg(fn, gpu_seconds, ref_cand, threads, reps=3, note=None):
+    """The oracle's -Ofast restatement of the reference loop on the same input:
+    one warm-up call, then the median of `reps` calls (whole calls: mesh build,
+    engine, reduction)."""
+    import numpy as np
+    first = fn()
+    walls = []
+    for _ in range(reps):
+        c0 = time.perf_counter()
+        fn()
+        walls.append(time.perf_counter() - c0)
+    wall = float(np.median(walls))
+    out = {"ms_per_call": wall * 1e3, "threads": threads, "kind": "port",
+           "reference_candidates": ref_cand, "candidates_per_s": ref_cand / wall,
+           "gpu_speedup": wall / gpu_seconds}
+    if note:
+        out["note"] = note
+    return first, out
+
+
+def bench_other_configs(dev, steps, cpu_threads=0):
     """GPU-only timings of the other hot-path rows on their full-size inputs.
 
     2M uniform points in L = 1000: npairs_xy_z with 16 log rp edges 0.1-30 and
@@ -150,11 +172,14 @@ def bench_other_configs(dev, steps):
     uniform halos in L = 1000 with log10 M ~ U[11, 15]: product-weighted marked
     sums with Zheng07 central weights (logMmin = 12, sigma = 0.3), forward plus
     backward to (logMmin, sigma), and marked_npairs_xy_z of the same weights.
-    The headline's clustered box: npairs_per_object_3d (its 200k-point
-    version), npairs_jackknife_3d with 125 spatial subvolumes, and tpcf with
-    analytic randoms.  Inputs are resident in HBM; each entry is the mean wall
+    The headline's ~1M-point clustered box: npairs_per_object_3d,
+    npairs_jackknife_3d with 125 spatial subvolumes, and tpcf with analytic
+    randoms.  Inputs are resident in HBM; each entry is the mean wall
     time of whole calls and, where given, the pair kernel's own evaluations per
-    second from HIP events.
+    second from HIP events.  With ``cpu_threads`` > 0 each entry also carries
+    ``cpu``: the oracle's -Ofast C restatement of the reference loop on the same
+    input (on a stated subsample where the full one would take minutes), with
+    results checked against the GPU's.
     """
     import numpy as np
     import torch
@@ -175,6 +200,8 @@ def bench_other_configs(dev, steps):
         return out, (time.perf_counter() - t0) / steps
 
     out = {}
+    from oracle import mesh as oracle
+    T = cpu_threads
     p3, _, L = mocks.config_points(3)
     t3 = torch.from_numpy(np.ascontiguousarray(p3)).to(dev)
     rp, pi = mocks.config_rbins(3), np.arange(0, 41, 1.0)
@@ -196,6 +223,14 @@ def bench_other_configs(dev, steps):
                                  + ("pi edges 0..40 step 1" if kind == E.HP_NPAIRS_XY_Z else "pi_max 40"),
                      "ms_per_call": secs * 1e3, "pair_kernel_ms": ks * 1e3, "kernel_evals": evals,
                      "kernel_evals_per_s": evals / ks, "fp64_peak_frac": 8.0 * evals / ks / 1e12 / FP64_PEAK_TFLOPS}
+        if T:
+            cand = reference_candidates(p3, (rp[-1], rp[-1], 40.0), L)
+            if kind == E.HP_NPAIRS_XY_Z:
+                fn = lambda: oracle.npairs_xy_z(p3, p3, rp, pi, period=L, num_threads=T, fast=True)
+            else:
+                fn = lambda: oracle.npairs_projected(p3, p3, rp, 40.0, period=L, num_threads=T, fast=True)
+            ref, out[name]["cpu"] = _cpu_leg(fn, secs, cand, T)
+            assert np.array_equal(ref, cnt.cpu().numpy())
     del t3, cols
 
     # per-object counts on the headline's clustered box: (N, 20) int64 rows
@@ -218,6 +253,11 @@ def bench_other_configs(dev, steps):
                     % len(p2),
         "ms_per_call": s_obj * 1e3, "pair_kernel_ms": ks * 1e3, "kernel_evals": evals,
         "kernel_evals_per_s": evals / ks, "fp64_peak_frac": 8.0 * evals / ks / 1e12 / FP64_PEAK_TFLOPS}
+    if T:
+        ref, out["npairs_per_object_3d"]["cpu"] = _cpu_leg(
+            lambda: oracle.npairs_per_object_3d(p2, p2, rb2, period=1000.0, num_threads=T, fast=True), s_obj,
+            reference_candidates(p2, rb2[-1], 1000.0), T)
+        assert np.array_equal(ref, rows.cpu().numpy())
     del t2, cols2, rows, res
 
     rng = np.random.default_rng([43, 4])
@@ -239,6 +279,15 @@ def bench_other_configs(dev, steps):
     out["marked_npairs_3d_product_fwd_bwd"] = {
         "workload": "1M uniform halos, L=1000, 20 log edges 0.1-50, Zheng07 weights, d/d(logMmin, sigma)",
         "ms_per_fwd_bwd": s_fb * 1e3}
+    if T:
+        w4h = mocks.zheng07_ncen(logm, torch.tensor(12.0, dtype=torch.float64, device=dev),
+                                 torch.tensor(0.3, dtype=torch.float64, device=dev)).cpu().numpy()
+        _, out["marked_npairs_3d_product_fwd_bwd"]["cpu"] = _cpu_leg(
+            lambda: oracle.marked_npairs_3d(p4, p4, rb, period=1000.0, weights1=w4h, weights2=w4h,
+                                            weight_func_id=1, num_threads=T, fast=True),
+            s_fb, reference_candidates(p4, rb[-1], 1000.0), T,
+            note="forward sums only: the reference has no backward, so gpu_speedup compares the GPU's "
+                 "forward + backward with the CPU's forward")
 
     # marked (rp, pi) sums of the same halos, product of the Zheng07 weights
     from halotools_amd.mock_observables import marked_npairs_xy_z
@@ -262,6 +311,13 @@ def bench_other_configs(dev, steps):
                     "weight_func_id 1 with Zheng07 weights",
         "ms_per_call": s_mx * 1e3, "pair_kernel_ms": ks * 1e3, "kernel_evals": evals,
         "kernel_evals_per_s": evals / ks, "fp64_peak_frac": 8.0 * evals / ks / 1e12 / FP64_PEAK_TFLOPS}
+    if T:
+        w4h = w4.cpu().numpy()
+        ref, out["marked_npairs_xy_z"]["cpu"] = _cpu_leg(
+            lambda: oracle.marked_npairs_xy_z(p4, p4, rp4, pi4, period=1000.0, weights1=w4h, weights2=w4h,
+                                              weight_func_id=1, num_threads=T, fast=True),
+            s_mx, reference_candidates(p4, (rp4[-1], rp4[-1], 40.0), 1000.0), T)
+        np.testing.assert_allclose(ref, wx.cpu().numpy(), rtol=1e-11)
     del t4, cols4, w4, res, wx
 
     # jackknife counts and the two-point caller layer on the headline's clustered box
@@ -276,13 +332,49 @@ def bench_other_configs(dev, steps):
         "workload": "%d-point NFW-clustered box (as the headline), L=1000, 20 log edges 0.1-50, "
                     "125 spatial subvolumes, unit weights, table (126, 20)" % len(p2c),
         "ms_per_call": s_jk * 1e3}
-    _, s_tp = timed(lambda: tpcf(t2c, rb2, period=1000.0))
+    if T:
+        # the reference engine walks the bins once per subsample (126 times per
+        # candidate pair): timed on every 8th point (1/64 of the candidates),
+        # with the GPU on the same subsample beside it
+        ps = np.ascontiguousarray(p2c[::8])
+        tg = tags.cpu().numpy()[::8]
+        ts, tgs = torch.from_numpy(ps).to(dev), torch.from_numpy(np.ascontiguousarray(tg)).to(dev)
+        gsub, s_jks = timed(lambda: npairs_jackknife_3d(ts, ts, rb2, tgs, tgs, 125, period=1000.0))
+        ref, leg = _cpu_leg(lambda: oracle.npairs_jackknife(ps, ps, rb2, tg, tg, 125, period=1000.0, num_threads=T,
+                                                            fast=True),
+                            s_jks, reference_candidates(ps, rb2[-1], 1000.0), T, reps=1,
+                            note="every 8th point of the box (%d points); GPU on the same subsample: %.3f ms"
+                                 % (len(ps), s_jks * 1e3))
+        assert np.array_equal(ref, gsub.cpu().numpy())
+        out["npairs_jackknife_3d"]["cpu"] = leg
+    xi, s_tp = timed(lambda: tpcf(t2c, rb2, period=1000.0))
     out["tpcf"] = {"workload": "xi(r) of the headline box, Natural estimator, analytic randoms "
                                "(DD on the device, estimator on the device)", "ms_per_call": s_tp * 1e3}
-    t3 = torch.from_numpy(np.ascontiguousarray(mocks.config_points(3)[0])).to(dev)
-    _, s_wp = timed(lambda: wp(t3, mocks.config_rbins(3), 40.0, period=1000.0))
+    if T:
+        from oracle import clustering as OC
+
+        def cpu_tpcf():
+            dd = np.diff(oracle.npairs_3d(p2c, p2c, rb2, period=1000.0, num_threads=T, fast=True))
+            dv = np.diff(4.0 / 3.0 * np.pi * rb2 ** 3)
+            n = len(p2c)
+            return OC.estimator(dd, None, dv * (n * n / 1e9), n, n, n, n, "Natural")
+        ref, out["tpcf"]["cpu"] = _cpu_leg(cpu_tpcf, s_tp, reference_candidates(p2c, rb2[-1], 1000.0), T)
+        np.testing.assert_allclose(ref, xi.cpu().numpy(), rtol=1e-12)
+    p3 = mocks.config_points(3)[0]
+    t3 = torch.from_numpy(np.ascontiguousarray(p3)).to(dev)
+    rb3 = mocks.config_rbins(3)
+    wpv, s_wp = timed(lambda: wp(t3, rb3, 40.0, period=1000.0))
     out["wp"] = {"workload": "wp(rp) of 2M uniform points, L=1000, 16 log rp edges 0.1-30, pi_max 40, "
                              "analytic randoms", "ms_per_call": s_wp * 1e3}
+    if T:
+        def cpu_wp():
+            dd = np.diff(np.diff(oracle.npairs_xy_z(p3, p3, rb3, np.array([0.0, 40.0]), period=1000.0,
+                                                    num_threads=T, fast=True), axis=0), axis=1)
+            dv = np.diff(np.diff(np.pi * np.outer(rb3 ** 2, 2.0 * np.array([0.0, 40.0])), axis=0), axis=1)
+            n = len(p3)
+            return 2.0 * OC.estimator(dd, None, dv * (n * n / 1e9), n, n, n, n, "Natural")[:, 0] * 40.0
+        ref, out["wp"]["cpu"] = _cpu_leg(cpu_wp, s_wp, reference_candidates(p3, (rb3[-1], rb3[-1], 40.0), 1000.0), T)
+        np.testing.assert_allclose(ref, wpv.cpu().numpy(), rtol=1e-12)
     return out
 
 
@@ -355,6 +447,16 @@ def main():
     counts = out.cpu().numpy()
     sec_per_step = (t1 - t0) / args.steps
 
+    # the same call with NumPy in and NumPy out (host-to-device copy of the
+    # points and the copy back of the counts included; not the headline value)
+    host_ms = []
+    for _ in range(max(3, args.steps // 4)):
+        h0 = time.perf_counter()
+        hc = npairs_3d(pts, pts, rbins, period=L)
+        host_ms.append(1e3 * (time.perf_counter() - h0))
+    assert np.array_equal(hc, counts)
+    numpy_io_ms = float(np.median(host_ms))
+
     # -------- dominant kernel: HIP events around the pair kernel on its stream
     cols = (t[:, 0].contiguous(), t[:, 1].contiguous(), t[:, 2].contiguous())
     mock = dict(n1=n, n2=n, period=[L, L, L], pbc=True)
@@ -384,17 +486,29 @@ def main():
     if not args.no_cpu_baseline:
         from oracle import mesh as oracle
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        c0 = time.perf_counter()
+        # one warm-up, then the median of 5 whole calls (mesh build + engine + reduction)
         ref = oracle.npairs_3d(pts, pts, rbins, period=L, num_threads=threads, fast=True)
-        c1 = time.perf_counter()
         assert np.array_equal(ref, counts), "GPU counts differ from the CPU reference loop"
-        cpu = {"value": ref_cand / (c1 - c0), "unit": "pair evals/s", "cores": threads, "kind": "port",
-               "sample": "the full workload once: reference mesh + engine loop restated in C (oracle/cpu_ref.c, "
-                         "-Ofast, no -march), cell-1 range split over %d threads; %.2f s wall; counts checked "
-                         "equal to the GPU's" % (threads, c1 - c0)}
+        walls = []
+        for _ in range(5):
+            c0 = time.perf_counter()
+            oracle.npairs_3d(pts, pts, rbins, period=L, num_threads=threads, fast=True)
+            walls.append(time.perf_counter() - c0)
+        wall = float(np.median(walls))
+        # single core: one call on the same input
+        c0 = time.perf_counter()
+        oracle.npairs_3d(pts, pts, rbins, period=L, num_threads=1, fast=True)
+        wall1 = time.perf_counter() - c0
+        cpu = {"value": ref_cand / wall, "unit": "pair evals/s", "cores": threads, "kind": "port",
+               "sample": "the full workload: reference mesh + engine loop restated in C (oracle/cpu_ref.c, "
+                         "-Ofast, no -march), cell-1 range split over %d threads; median of 5 calls after a "
+                         "warm-up, %.2f s each; counts checked equal to the GPU's" % (threads, wall),
+               "single_core_s": wall1, "single_core_value": ref_cand / wall1}
 
     batched = None if args.no_batched else bench_batched(dev, max(3, args.steps // 4))
-    others = None if args.no_others else bench_other_configs(dev, max(3, args.steps // 4))
+    others = None if args.no_others else bench_other_configs(
+        dev, max(3, args.steps // 4),
+        cpu_threads=0 if args.no_cpu_baseline else (args.cpu_threads or min(os.cpu_count() or 1, 16)))
 
     line = {
         "metric": "pair evals/s & %FP64 peak (npairs_3d, 1M pts); batched fwd+bwd ms per mock",
@@ -419,6 +533,7 @@ def main():
         "pair_kernel_ms": kern_s * 1e3,
         "fp64_peak_frac": achieved / FP64_PEAK_TFLOPS,
         "cell_build": _build_rate(n, build_s),
+        "numpy_in_out_ms_per_call": numpy_io_ms,
         # compute-bound: FP64 vector ALU (a distance threshold, not a contraction, so no MFMA);
         # MI355X's FP64 peak is 78.6 TFLOP/s for vector and matrix alike
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -394,7 +394,8 @@ def marked_npairs_xy_z(sample1, sample2, rp_bins, pi_bins, period=None, weights1
 
 
 def npairs_jackknife(sample1, sample2, rp_bins, jtags1, jtags2, N_samples, pi_bins=None, period=None,
-                     weights1=None, weights2=None, num_threads=1, approx_cell1_size=None, approx_cell2_size=None):
+                     weights1=None, weights2=None, num_threads=1, approx_cell1_size=None, approx_cell2_size=None,
+                     fast=False):
     """Oracle for PC/npairs_jackknife_3d.py:24-95 (pi_bins None) and
     PC/npairs_jackknife_xy_z.py:23-199: (N_samples+1, K) or (N_samples+1, K, G)
     float64 (engine loops PC/cpairs/npairs_jackknife_3d_engine.pyx:98-236,
@@ -418,7 +419,7 @@ def npairs_jackknife(sample1, sample2, rp_bins, jtags1, jtags2, N_samples, pi_bi
     G = 1 if pi_bins is None else len(pi_bins)
     out = np.zeros((N_samples + 1, len(rp_bins), G), dtype=np.float64)
     mc = mesh.as_c()
-    rc = load().ref_npairs_jackknife(ctypes.byref(mc), *_ptrs(cols), w1s.ctypes.data, w2s.ctypes.data,
+    rc = load(fast).ref_npairs_jackknife(ctypes.byref(mc), *_ptrs(cols), w1s.ctypes.data, w2s.ctypes.data,
                                      j1s.ctypes.data, j2s.ctypes.data, int(N_samples), rp_bins.ctypes.data,
                                      len(rp_bins), None if pi_bins is None else pi_bins.ctypes.data, G,
                                      int(num_threads), out.ctypes.data)
