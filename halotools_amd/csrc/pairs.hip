@@ -149,6 +149,9 @@ constexpr int kBatch = HP_BATCH;  // stream points per scalar-load wait
 #ifndef HP_SPLIT_CHUNK
 #define HP_SPLIT_CHUNK 2
 #endif
+#ifndef HP_SPLIT_XY
+#define HP_SPLIT_XY 1
+#endif
 #ifndef HP_COUNT_WAVES
 #define HP_COUNT_WAVES 6  // waves per SIMD the count kernels' registers are held to
 #endif
@@ -536,7 +539,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
         }
         evals += (uint64_t)cnt * (uint64_t)nv;
         uint32_t a[B];
-        if (HP_SPLIT && !XY && !MULTI) {
+        if (HP_SPLIT && (!XY || HP_SPLIT_XY) && !MULTI) {
             // per chunk of C slots: the distances first, then all of their
             // bucket reads, then the compares -- one LDS wait per chunk
             // instead of one per slot (C bounds the live registers)
@@ -549,19 +552,24 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                     const int u = h + c;
                     double dx, dy, dz;
                     sep(bx[u], by[u], bz[u], rshx, rshy, rshz, rsxs, rsys, rszs, rax, ray, raz, dx, dy, dz);
-                    if (MODE == kModeProj) {
+                    if (MODE == kModeProj || XY) {
                         dq[c] = dx * dx + dy * dy;
                         dz2[c] = dz * dz;
                     } else {
                         dq[c] = dx * dx + dy * dy + dz * dz;
                     }
                 }
-                uint4 ent[C];
+                uint4 ent[C], enp[C];
 #pragma unroll
                 for (int c = 0; c < C; c++) {
                     const uint32_t hi = (uint32_t)__double2hiint(dq[c]);
                     const uint32_t b = min(__builtin_elementwise_sub_sat(hi, bshr) >> shr, nbr1);
                     ent[c] = *reinterpret_cast<const uint4*>(s_tr + b);
+                    if (XY) {
+                        const uint32_t hp = (uint32_t)__double2hiint(dz2[c]);
+                        const uint32_t bp = min(__builtin_elementwise_sub_sat(hp, bshp) >> shp, nbp1);
+                        enp[c] = *reinterpret_cast<const uint4*>(s_tp + bp);
+                    }
                 }
 #pragma unroll
                 for (int c = 0; c < C; c++) {
@@ -569,6 +577,12 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                     const double edge = __hiloint2double((int)ent[c].y, (int)ent[c].x);
                     uint32_t v = ent[c].z + (edge < dq[c] ? step_r : 0u);
                     if (MODE == kModeProj) v = (dz2[c] <= pimax2) ? v : trash_r;
+                    if (XY) {
+                        // outside (rp_max, pi_max): trash_r, which skips the atomics
+                        const double ep = __hiloint2double((int)enp[c].y, (int)enp[c].x);
+                        const uint32_t ap = enp[c].z + (ep < dz2[c] ? cell_bytes : 0u);
+                        v = (v < trash_r && ap < trash_p) ? v + ap : trash_r;
+                    }
                     a[u] = (u < nv) ? v : trash_r;
                 }
             }
