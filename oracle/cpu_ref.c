@@ -37,6 +37,14 @@
 /* 3: window one cell too narrow on the high side; 4: rbins not squared.     */
 static int g_mutation = 0;
 void ref_set_mutation(int m) { g_mutation = m; }
+
+/* compensated summation switch (test infrastructure): marked sums accumulate
+ * with Neumaier's compensated sum instead of the reference's plain serial sum,
+ * giving a reference value accurate to ~1 ulp for large inputs, where any two
+ * summation orders of ~1e9 terms (the reference's among them) differ by more
+ * than 1e-12 relative.  Only the exact build honours it. */
+static int g_compensated = 0;
+void ref_set_compensated(int c) { g_compensated = c; }
 int ref_get_mutation(void) { return g_mutation; }
 
 /* Mesh description handed over by oracle/mesh.py.  Field meaning follows the
@@ -146,7 +154,22 @@ typedef struct ref_job {
     double pimax2;      /* npairs_projected               */
     int64_t* counts;    /* int64 output (K) or (K*G)      */
     double* wcounts;    /* float64 output (K)             */
+    double* wcomp;      /* compensation terms of wcounts (g_compensated) */
 } ref_job;
+
+/* s += x, optionally compensated (Neumaier 1974, "Rundungsfehleranalyse einiger
+ * Verfahren zur Summation endlicher Summen"): the rounding error of each
+ * addition is carried in c and added back at the end. */
+static inline void acc_add(double* s, double* c, double x) {
+    if (!c) {
+        *s += x;
+        return;
+    }
+    const double t = *s + x;
+    if (fabs(*s) >= fabs(x)) *c += (*s - t) + x;
+    else *c += (x - t) + *s;
+    *s = t;
+}
 
 /* jackknife weight of a pair for subsample s (0 = whole sample):
  * PC/cpairs/npairs_jackknife_3d_engine.pyx:239-287 */
@@ -232,7 +255,7 @@ static void run_cells(const ref_job* J, int64_t first, int64_t last) {
                                 const double weight = wfunc_from_id(J->wfunc_id)(J->w1 + i * J->nw, J->w2 + j * J->nw);
                                 int k = K - 1;
                                 while (bin_pass(dsq, J->r2[k])) {
-                                    J->wcounts[k] += weight;
+                                    acc_add(&J->wcounts[k], J->wcomp ? &J->wcomp[k] : NULL, weight);
                                     k = k - 1;
                                     if (k < 0) break;
                                 }
@@ -245,7 +268,8 @@ static void run_cells(const ref_job* J, int64_t first, int64_t last) {
                                 while (bin_pass(dxy_sq, J->r2[k])) {
                                     int g = G - 1;
                                     while (bin_pass(dz_sq, J->pi2[g])) {
-                                        J->wcounts[(int64_t)k * G + g] += weight;
+                                        acc_add(&J->wcounts[(int64_t)k * G + g],
+                                                J->wcomp ? &J->wcomp[(int64_t)k * G + g] : NULL, weight);
                                         g = g - 1;
                                         if (g < 0) break;
                                     }
@@ -346,8 +370,15 @@ static int run_threads(const ref_job* J, int nthreads) {
     /* per-object rows of different cell-1 chunks are disjoint: workers share
      * the output (the reference sums per-worker arrays that are zero elsewhere) */
     const int shared_out = (J->kind == KIND_OBJ);
+    const int comp = g_compensated && (J->kind == KIND_MARKED || J->kind == KIND_MARKED_XYZ);
     if (nthreads <= 1 || ncells <= 1) {
-        run_cells(J, first, last);
+        ref_job one = *J;
+        one.wcomp = comp ? (double*)calloc((size_t)outlen, sizeof(double)) : NULL;
+        run_cells(&one, first, last);
+        if (comp) {
+            for (int64_t k = 0; k < outlen; k++) J->wcounts[k] += one.wcomp[k];
+            free(one.wcomp);
+        }
         return 0;
     }
     if (nthreads > ncells) nthreads = (int)ncells;
@@ -362,6 +393,7 @@ static int run_threads(const ref_job* J, int nthreads) {
         args[t].job = *J;
         if (fp_out) {
             args[t].job.wcounts = (double*)calloc((size_t)outlen, sizeof(double));
+            args[t].job.wcomp = comp ? (double*)calloc((size_t)outlen, sizeof(double)) : NULL;
         } else if (!shared_out) {
             args[t].job.counts = (int64_t*)calloc((size_t)outlen, sizeof(int64_t));
         }
@@ -371,14 +403,26 @@ static int run_threads(const ref_job* J, int nthreads) {
         pthread_create(&th[t], NULL, worker_main, &args[t]);
     }
     for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    double* ctot = comp ? (double*)calloc((size_t)outlen, sizeof(double)) : NULL;
     for (int t = 0; t < nthreads; t++) {
         if (shared_out) continue;
         for (int64_t k = 0; k < outlen; k++) {
-            if (fp_out) J->wcounts[k] += args[t].job.wcounts[k];
-            else J->counts[k] += args[t].job.counts[k];
+            if (comp) {
+                acc_add(&J->wcounts[k], &ctot[k], args[t].job.wcounts[k]);
+                acc_add(&J->wcounts[k], &ctot[k], args[t].job.wcomp[k]);
+            } else if (fp_out) {
+                J->wcounts[k] += args[t].job.wcounts[k];
+            } else {
+                J->counts[k] += args[t].job.counts[k];
+            }
         }
         free(args[t].job.counts);
         free(args[t].job.wcounts);
+        free(args[t].job.wcomp);
+    }
+    if (comp) {
+        for (int64_t k = 0; k < outlen; k++) J->wcounts[k] += ctot[k];
+        free(ctot);
     }
     free(args);
     free(th);

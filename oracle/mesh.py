@@ -82,10 +82,16 @@ def load(fast=False):
     lib.ref_candidate_pairs.argtypes = [ctypes.POINTER(_RefMeshC)]
     lib.ref_candidate_pairs.restype = ctypes.c_int64
     lib.ref_set_mutation.argtypes = [I]
+    lib.ref_set_compensated.argtypes = [I]
     lib.ref_wfunc.argtypes = [I, P, P]
     lib.ref_wfunc.restype = D
     _LIBS[name] = lib
     return lib
+
+
+def set_compensated(on):
+    """Compensated (Neumaier) summation of the exact oracle's marked sums (tests only)."""
+    load().ref_set_compensated(1 if on else 0)
 
 
 def set_mutation(m):

@@ -246,6 +246,22 @@ def test_planted_mistakes_are_caught(mutation):
     assert fails, "mutation %d went unnoticed" % mutation
 
 
+def test_compensated_marked_sums_equal_fsum():
+    # the compensated mode (a reference value for large inputs) against the
+    # brute force's correctly rounded math.fsum, with threads and without
+    s = random_sample(800, seed=2)
+    w = np.random.RandomState(3).random_sample(800) - 0.3
+    rb = np.array([0.05, 0.1, 0.2, 0.3])
+    ORC.set_compensated(True)
+    try:
+        for nt in (1, 4):
+            a = ORC.marked_npairs_3d(s, s, rb, period=1.0, weights1=w, weights2=w, weight_func_id=1, num_threads=nt)
+            np.testing.assert_allclose(a, brute.wnpairs(s, s, rb, period=1.0, weights1=w, weights2=w),
+                                       rtol=1e-15, atol=0)
+    finally:
+        ORC.set_compensated(False)
+
+
 def test_per_object_rows_add_up_to_npairs_3d():
     # every row set sums to the aggregate counts, whatever the mesh or threads
     s1, s2 = random_sample(700, seed=11), random_sample(500, seed=12)
