@@ -35,6 +35,10 @@ def _as_weights(w, sample, name):
         w = np.asarray(w).astype("float64")
     if _n(w) != _n(sample):
         raise HalotoolsError("%s should have same len as %s" % (name, "sample" + name[-1]))
+    if w.ndim != 1:
+        # the reference's engine takes one weight per point (a 1-D buffer) and
+        # rejects other shapes when it casts them
+        raise ValueError("Buffer has wrong number of dimensions (expected 1, got %d)" % w.ndim)
     return w
 
 

@@ -101,6 +101,12 @@ def test_process_weights_and_tags_messages():
         _npairs_jackknife_3d_process_weights_jtags(s1, s2, w1, w2, np.ones(10, dtype=int), t2, 5)
 
 
+def test_two_dimensional_weights_rejected():
+    s1, s2, w1, w2, t1, t2 = _w_t()
+    with pytest.raises(ValueError):
+        _npairs_jackknife_3d_process_weights_jtags(s1, s2, np.ones((10, 2)), w2, t1, t2, 5)
+
+
 def test_drop_in_argument_errors_before_any_gpu_work():
     s1, s2, w1, w2, t1, t2 = _w_t(100, 100)
     with pytest.raises(HalotoolsError):
