@@ -627,8 +627,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
         }
         if (MODE == kModeTan || MODE == kModeJac || JK) {
             // weights of the whole batch in one scalar-load wait; most
-            // evaluated pairs lie beyond the largest edge, and their lanes skip
-            // the FP64 atomics instead of adding to the trash row
+            // evaluated pairs lie beyond the largest edge
             double bw[B], bt[B];
 #pragma unroll
             for (int u = 0; u < B; u++) {
@@ -651,7 +650,9 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
             }
 #pragma unroll
             for (int u = 0; u < B; u++) {
-                if (a[u] == trash_r) continue;
+                // the tangent pass's three atomics are worth a branch; the
+                // Jacobian's one is not (it adds into the trash row instead)
+                if (a[u] == trash_r && (JK || MODE == kModeTan)) continue;
                 if (JK) {
                     jk_add(a[u], rl0 * bw[u], jt[u], js[u]);
                 } else if (MODE == kModeTan) {
