@@ -181,6 +181,7 @@ struct PairArgs {
     int ring_off;       // byte offset of the survivor ring within a wave's LDS
     int tab_bytes;      // LDS bytes of the bucket tables, ahead of the wave areas
     int jrow;           // jackknife: bytes of one subsample row of the wave's table
+    int jr0_off;        // 3-D jackknife: byte offset of the all-pairs rows in 16 lane columns
 };
 
 // Survivor ring per wave: up to kBatch-1 leftover entries plus one 64-point chunk.
@@ -463,7 +464,32 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                                __HIP_MEMORY_SCOPE_AGENT);
     };
     // one pair in cell a: v = w1 w2 (times 2 for an unshifted pair under symmetry)
+    // A group whose points all carry one tag t (the common case for spatial
+    // subsamples; 3-D bins only) keeps the all-pairs row in 16 lane columns,
+    // where no two lanes of one LDS pass collide, and records only the shares
+    // its cross-tag pairs give their partners' tags (half of each, in their
+    // rows; what goes straight to HBM is also summed in row 1, t's own row,
+    // unused otherwise): the loss of t is then the all-pairs row minus them.
+    bool jut = false;
+    if (JK && !XY) jut = (ns == 1) && ((uint64_t)__ballot(act && lslot != 1) == 0);
+    const uint32_t r0col = (uint32_t)(lane & 15) * 8u;
     auto jk_add = [&](uint32_t a, double v, int ts, int ss) __attribute__((always_inline)) {
+        if (!XY && jut) {
+            __hip_atomic_fetch_add(reinterpret_cast<double*>(wsm + A.jr0_off + a * 16u + r0col), v,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (ltag != ts) {
+                const double sv = 0.5 * v;
+                if (ss > 0) {
+                    __hip_atomic_fetch_add(reinterpret_cast<double*>(wsm + ss * A.jrow + a), sv, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+                    jk_global(ts, a, sv);
+                    __hip_atomic_fetch_add(reinterpret_cast<double*>(wsm + A.jrow + a), sv, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+            return;
+        }
         __hip_atomic_fetch_add(reinterpret_cast<double*>(wsm + a), v, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
         const bool same_t = (ltag == ts);
@@ -904,7 +930,32 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
             row[s] = run;
         }
     }
-    if (JK) {
+    if (JK && !XY && jut) {
+        // uniform-tag group: row 0 = the 16 lane columns; t's loss = row 0
+        // minus the shares recorded in rows 1..ns; partners' rows as they are
+        double* dst = L.jacc + (int64_t)m * (L.nsamp + 1) * nbj;
+        for (int k = lane; k < Kr; k += 64) {
+            const double* r0 = reinterpret_cast<const double*>(wsm + A.jr0_off + k * 128);
+            double tot = 0.0;
+            for (int c = 0; c < 16; c++) tot += r0[c];
+            double xs = 0.0;
+            for (int r = 1; r <= ns; r++) xs += reinterpret_cast<const double*>(wsm + r * A.jrow)[k];
+            if (tot != 0.0) __hip_atomic_fetch_add(dst + k, tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (tot - xs != 0.0)
+                __hip_atomic_fetch_add(dst + (int64_t)ltag * nbj + k, tot - xs, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            for (int r = 2; r <= ns; r++) {
+                int tag = 0;
+#pragma unroll
+                for (int j = 0; j < kJackSlots; j++)
+                    if (j == r - 1) tag = stag[j];
+                const double val = reinterpret_cast<const double*>(wsm + r * A.jrow)[k];
+                if (val != 0.0)
+                    __hip_atomic_fetch_add(dst + (int64_t)tag * nbj + k, val, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    } else if (JK) {
         // the wave's rows into the mock's table (row 0, then each tag's row)
         const int cells = XY ? (Kr + 1) * (Kp + 1) : Kr + 1;
         for (int r = 0; r <= ns; r++) {
@@ -990,6 +1041,7 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
     A.cnt_off = 0;
     A.tan_off = 0;
     A.jrow = 0;
+    A.jr0_off = 0;
     int wave_bytes = 0;
     switch (L.mode) {
         case kModeCount3D:
@@ -1004,6 +1056,11 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
             // row 0 plus one row per subsample tag the wave keeps in LDS
             A.jrow = (Kr + 1) * (L.mode == kModeJackXY ? Kp + 1 : 1) * 8;
             wave_bytes = (L.jslots + 1) * A.jrow;
+            if (L.mode == kModeJack) {
+                // all-pairs rows in 16 lane columns for uniform-tag groups
+                A.jr0_off = wave_bytes;
+                wave_bytes += (Kr + 1) * 16 * 8;
+            }
             break;
         case kModeObj: wave_bytes = (Kr + 1) * 64 * 4; break;
         case kModeJac:
