@@ -150,16 +150,81 @@ __global__ void __launch_bounds__(256) k_reduce_plain(const uint64_t* __restrict
 
 static int64_t nchunk_of(int64_t max_groups) { return (max_groups + kChunk - 1) / kChunk; }
 
+// Wide histograms (the (rp, pi) cells of marked xy_z, hundreds of cells per
+// group): stage 1 takes chunks of kChunkW groups, one cell per thread, so a
+// launch has (groups / 64) x (cells / 256) blocks instead of groups / 512;
+// stage 2 sums a mock's chunk rows in four fixed stripes per cell, then the
+// stripes in order; the 2-D prefix runs once per mock.
+constexpr int kChunkW = 64;
+constexpr int kWideCells = 64;  // histograms above this many cells take the wide path
+static int64_t nchunk_w(int64_t max_groups) { return (max_groups + kChunkW - 1) / kChunkW; }
+
+__global__ void __launch_bounds__(256) k_reduce_w1(const double* __restrict__ partial, int nb,
+                                                   const MockGeom* __restrict__ geom, int side, int nchunk_max,
+                                                   double* __restrict__ s1) {
+    const int m = blockIdx.y, c = blockIdx.x;
+    const int b = blockIdx.z * 256 + threadIdx.x;
+    const int64_t go = geom[m].group_off[side], ng = geom[m].ngroups[side];
+    const int64_t r0 = (int64_t)c * kChunkW;
+    if (b >= nb || r0 >= ng) return;
+    const int64_t r1 = r0 + kChunkW < ng ? r0 + kChunkW : ng;
+    double acc = 0.0;
+    for (int64_t r = r0; r < r1; r++) acc += partial[(go + r) * nb + b];
+    s1[((int64_t)m * nchunk_max + c) * nb + b] = acc;
+}
+
+__global__ void __launch_bounds__(256) k_reduce_w2(const double* __restrict__ s1, int nb,
+                                                   const MockGeom* __restrict__ geom, int side, int nchunk_max,
+                                                   double* __restrict__ out) {
+    __shared__ double sh[4][64];
+    const int m = blockIdx.y;
+    const int bl = threadIdx.x & 63, stripe = threadIdx.x >> 6;
+    const int b = blockIdx.x * 64 + bl;
+    const int64_t ng = geom[m].ngroups[side];
+    const int nchunks = (int)((ng + kChunkW - 1) / kChunkW);
+    double acc = 0.0;
+    if (b < nb)
+        for (int c = stripe; c < nchunks; c += 4) acc += s1[((int64_t)m * nchunk_max + c) * nb + b];
+    sh[stripe][bl] = acc;
+    __syncthreads();
+    if (stripe == 0 && b < nb) out[(int64_t)m * nb + b] = ((sh[0][bl] + sh[1][bl]) + sh[2][bl]) + sh[3][bl];
+}
+
+// inclusive 2-D prefix of each mock's [nr][np] sums (over pi, then over rp)
+__global__ void __launch_bounds__(256) k_prefix_2d(double* __restrict__ out, int nr, int np) {
+    double* o = out + (int64_t)blockIdx.x * nr * np;
+    for (int k = threadIdx.x; k < nr; k += blockDim.x) {
+        double run = 0.0;
+        for (int q = 0; q < np; q++) { run += o[k * np + q]; o[k * np + q] = run; }
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int q = threadIdx.x; q < np; q += blockDim.x) {
+        double run = 0.0;
+        for (int k = 0; k < nr; k++) { run += o[k * np + q]; o[k * np + q] = run; }
+    }
+}
+
 int64_t reduce_scratch_elems(int64_t max_groups_per_mock, int nmocks, int nb) {
-    const int64_t nc = nchunk_of(max_groups_per_mock) > 0 ? nchunk_of(max_groups_per_mock) : 1;
+    const int64_t g = max_groups_per_mock > 0 ? max_groups_per_mock : 1;
+    const int64_t nc = nb > kWideCells ? nchunk_w(g) : nchunk_of(g);
     return nc * nmocks * (int64_t)nb;
 }
 
 // FP64 partial histograms [groups][nr] (2-D: [groups][nr * np]) -> cumulative sums per mock
 int launch_reduce_sums_s(const double* partial, int nr, int np, const MockGeom* geom, int nmocks, int side,
                          int64_t max_groups, double* scratch, double* sums, hipStream_t s) {
-    const int nc = (int)(nchunk_of(max_groups) > 0 ? nchunk_of(max_groups) : 1);
     const int nb = np > 0 ? nr * np : nr;
+    if (np > 0 && nb > kWideCells) {
+        const int nc = (int)(nchunk_w(max_groups) > 0 ? nchunk_w(max_groups) : 1);
+        hipLaunchKernelGGL(k_reduce_w1, dim3(nc, nmocks, (nb + 255) / 256), dim3(256), 0, s, partial, nb, geom, side,
+                           nc, scratch);
+        hipLaunchKernelGGL(k_reduce_w2, dim3((nb + 63) / 64, nmocks), dim3(256), 0, s, scratch, nb, geom, side, nc,
+                           sums);
+        hipLaunchKernelGGL(k_prefix_2d, dim3(nmocks), dim3(256), 0, s, sums, nr, np);
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
+    const int nc = (int)(nchunk_of(max_groups) > 0 ? nchunk_of(max_groups) : 1);
     hipLaunchKernelGGL(k_reduce_s1<double>, dim3(nc, nmocks), dim3(256), 0, s, partial, nb, geom, side, nc, scratch);
     hipLaunchKernelGGL((k_reduce_s2<double, double>), dim3(nmocks), dim3(256), 0, s, scratch, nr, np, geom, side, nc,
                        sums);
