@@ -36,10 +36,38 @@ def same_array(a, b):
     return False
 
 
+def xyz_rows(cols):
+    """The C-ordered float64 (N, 3) array whose columns ``cols`` are, or None.
+
+    ``sample[:, 0], sample[:, 1], sample[:, 2]`` of a C-contiguous float64
+    sample are column views 8 bytes apart with a 24-byte stride; the rows
+    they come from are then one contiguous block that can be copied as is.
+    """
+    if not all(isinstance(c, np.ndarray) for c in cols):
+        return None
+    c0 = cols[0]
+    n = len(c0)
+    if n == 0 or any(c.dtype != np.float64 or c.ndim != 1 or len(c) != n for c in cols):
+        return None
+    if any(c.strides != (24,) for c in cols):
+        return None
+    p0 = c0.__array_interface__["data"][0]
+    if any(cols[k].__array_interface__["data"][0] != p0 + 8 * k for k in range(3)):
+        return None
+    if not all(c.flags.writeable for c in cols):
+        return None
+    return np.lib.stride_tricks.as_strided(c0, shape=(n, 3), strides=(24, 8))
+
+
 def to_device_columns(cols, device):
     """Three coordinate columns (NumPy or torch, any real dtype) -> contiguous float64 device tensors."""
     if all(isinstance(c, torch.Tensor) for c in cols):
         return tuple(c.to(device=device, dtype=torch.float64).contiguous() for c in cols)
+    base = xyz_rows(cols)
+    if base is not None:
+        # one copy of the caller's (N, 3) rows; the columns are split on the device
+        t = torch.from_numpy(base).to(device, non_blocking=False)
+        return tuple(t[:, k].contiguous() for k in range(3))
     host = np.empty((3, len(cols[0])), dtype=np.float64)
     for k in range(3):
         host[k] = np.asarray(cols[k])

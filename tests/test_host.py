@@ -288,3 +288,23 @@ def test_segment_sum_scratch_and_validation():
     assert "non-decreasing" in lib.hp_last_error().decode()
     out = ctypes.c_double()
     assert lib.hp_segment_sum(None, offs, 3, ctypes.byref(out), None, 0, None) < 0
+
+
+def test_xyz_rows_detects_one_block_of_rows():
+    # NumPy (N, 3) samples go to the device as one copy when their columns are
+    # the 24-byte-strided views of one C-ordered float64 block
+    from halotools_amd import _device as D
+    s = np.random.RandomState(3).random_sample((1000, 3))
+    rows = D.xyz_rows(P.columns(s))
+    assert rows is not None and rows.shape == (1000, 3) and np.array_equal(rows, s)
+    assert np.shares_memory(rows, s)
+    # anything else takes the column-by-column path
+    assert D.xyz_rows(P.columns(np.asfortranarray(s))) is None
+    assert D.xyz_rows(P.columns(s.astype(np.float32))) is None
+    assert D.xyz_rows(P.columns(np.random.random_sample((10, 4)))) is None
+    assert D.xyz_rows((s[:, 1], s[:, 0], s[:, 2])) is None
+    assert D.xyz_rows(P.columns(s[::2])) is None
+    assert D.xyz_rows(P.columns(np.empty((0, 3)))) is None
+    ro = s.copy()
+    ro.flags.writeable = False
+    assert D.xyz_rows(P.columns(ro)) is None
