@@ -412,8 +412,8 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X (the multi-GPU "
-              "sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        print("bench.py measures one GPU: the pair counters run on a single MI355X; sharding "
+              "across GPUs is not implemented, so --gpus must be 1.", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
