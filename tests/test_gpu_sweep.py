@@ -5,9 +5,13 @@ points), box (cubic or not, periodic or enclosed), bin edges (1 to 80 edges,
 so both the per-lane and the shared per-wave count histograms run), cell-size
 hints and point layout (uniform, or clumped so that lane groups see dense
 and empty cells), and runs npairs_3d, npairs_projected, npairs_xy_z,
-npairs_per_object_3d and the product-marked sums through the drop-in API
-against the oracle: counts bit-exact, marked sums within 1e-12 relative.
+npairs_per_object_3d, the product-marked sums (3-D and (rp, pi)) and the
+jackknife tables (3-D and (rp, pi), random subsample tags) through the
+drop-in API against the oracle: counts and unit-weight jackknife tables
+bit-exact, marked sums within 1e-12 relative.
 """
+import warnings
+
 import numpy as np
 import pytest
 
@@ -78,3 +82,35 @@ def test_marked_sweep(mo, seed):
         ref = ORC.marked_npairs_3d(a, b, rb, period=period, weights1=wa, weights2=wb, weight_func_id=1,
                                    num_threads=4)
         np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_marked_xy_z_sweep(mo, seed):
+    s1, s2, rb, pi, period, cell = _case(seed)
+    rng = np.random.RandomState(3000 + seed)
+    w1 = rng.uniform(0, 1, len(s1))
+    w2 = rng.uniform(0, 1, len(s2))
+    for a, b, wa, wb in ((s1, s2, w1, w2), (s1, s1, w1, w1)):
+        got = mo.marked_npairs_xy_z(a, b, rb, pi, period=period, weights1=wa, weights2=wb, weight_func_id=1)
+        ref = ORC.marked_npairs_xy_z(a, b, rb, pi, period=period, weights1=wa, weights2=wb, weight_func_id=1,
+                                     num_threads=4)
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_jackknife_sweep(mo, seed):
+    # random tags: most waves meet several subsamples (LDS rows, then HBM)
+    s1, s2, rb, pi, period, cell = _case(seed)
+    rng = np.random.RandomState(4000 + seed)
+    ns = int(rng.choice([2, 5, 27, 40]))
+    t1 = rng.randint(1, ns + 1, len(s1))
+    t2 = rng.randint(1, ns + 1, len(s2))
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")  # samples without every tag warn, as in the reference
+        for a, b, ta, tb in ((s1, s2, t1, t2), (s1, s1, t1, t1)):
+            got = mo.npairs_jackknife_3d(a, b, rb, ta, tb, ns, period=period)
+            ref = ORC.npairs_jackknife(a, b, rb, ta, tb, ns, period=period, num_threads=4)
+            assert np.array_equal(got, ref), "jackknife 3-D"
+            got = mo.npairs_jackknife_xy_z(a, b, rb, pi, ta, tb, ns, period=period)
+            ref = ORC.npairs_jackknife(a, b, rb, ta, tb, ns, pi_bins=pi, period=period, num_threads=4)
+            assert np.array_equal(got, ref), "jackknife xy_z"
