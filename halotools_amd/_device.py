@@ -59,15 +59,41 @@ def xyz_rows(cols):
     return np.lib.stride_tricks.as_strided(c0, shape=(n, 3), strides=(24, 8))
 
 
+def tensor_xyz_rows(cols):
+    """The float64 (N, 3) tensor whose columns ``cols`` are (as a view), or None."""
+    if not all(isinstance(c, torch.Tensor) for c in cols):
+        return None
+    c0 = cols[0]
+    if c0.dim() != 1 or c0.shape[0] == 0:
+        return None
+    n = c0.shape[0]
+    if any(c.dtype != torch.float64 or c.dim() != 1 or c.shape[0] != n or c.device != c0.device for c in cols):
+        return None
+    if any(c.stride() != (3,) for c in cols):
+        return None
+    p0 = c0.data_ptr()
+    if any(cols[k].data_ptr() != p0 + 8 * k for k in range(3)):
+        return None
+    return torch.as_strided(c0, (n, 3), (3, 1))
+
+
+def _split_rows(rows):
+    """(N, 3) float64 device rows -> contiguous x, y, z (one transposing copy)."""
+    t = rows.t().contiguous()
+    return t[0], t[1], t[2]
+
+
 def to_device_columns(cols, device):
     """Three coordinate columns (NumPy or torch, any real dtype) -> contiguous float64 device tensors."""
     if all(isinstance(c, torch.Tensor) for c in cols):
+        rows = tensor_xyz_rows(cols)
+        if rows is not None:
+            return _split_rows(rows.to(device))
         return tuple(c.to(device=device, dtype=torch.float64).contiguous() for c in cols)
     base = xyz_rows(cols)
     if base is not None:
         # one copy of the caller's (N, 3) rows; the columns are split on the device
-        t = torch.from_numpy(base).to(device, non_blocking=False)
-        return tuple(t[:, k].contiguous() for k in range(3))
+        return _split_rows(torch.from_numpy(base).to(device, non_blocking=False))
     host = np.empty((3, len(cols[0])), dtype=np.float64)
     for k in range(3):
         host[k] = np.asarray(cols[k])

@@ -308,3 +308,21 @@ def test_xyz_rows_detects_one_block_of_rows():
     ro = s.copy()
     ro.flags.writeable = False
     assert D.xyz_rows(P.columns(ro)) is None
+
+
+def test_tensor_xyz_rows_detects_one_block_of_rows():
+    import torch
+    from halotools_amd import _device as D
+    t = torch.from_numpy(np.random.RandomState(4).random_sample((500, 3)))
+    rows = D.tensor_xyz_rows(P.columns(t))
+    assert rows is not None and tuple(rows.shape) == (500, 3) and torch.equal(rows, t)
+    assert rows.data_ptr() == t.data_ptr()
+    x, y, z = D._split_rows(rows)
+    assert torch.equal(x, t[:, 0]) and torch.equal(y, t[:, 1]) and torch.equal(z, t[:, 2])
+    assert x.is_contiguous() and z.is_contiguous()
+    assert D.tensor_xyz_rows(P.columns(t.float())) is None
+    assert D.tensor_xyz_rows(P.columns(t[::2])) is None
+    assert D.tensor_xyz_rows((t[:, 2], t[:, 1], t[:, 0])) is None
+    assert D.tensor_xyz_rows(P.columns(torch.zeros((10, 4), dtype=torch.float64))) is None
+    assert D.tensor_xyz_rows(P.columns(t.t().contiguous().t())) is None
+    assert D.tensor_xyz_rows(P.columns(torch.zeros((0, 3), dtype=torch.float64))) is None
