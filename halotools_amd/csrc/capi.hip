@@ -145,6 +145,7 @@ struct Plan {
     size_t o_pcount = 0, o_psum = 0, o_pevals = 0, o_red = 0, o_status = 0;
     bool elim = false;
     size_t o_corr = 0, o_rows_tmp = 0;
+    size_t o_hpos = 0, o_hrank = 0;  // symmetric auto-counts: Hilbert positions and ranks
     bool sym = false;  // auto-count (unweighted or tangent): symmetric evaluation
     int crep = 1;
     bool tangent = false;
@@ -398,9 +399,13 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
         P.o_corr = take((size_t)std::max<int64_t>(P.n[0], 1) * (size_t)P.Kr * 8);
         if (!P.rows1) P.o_rows_tmp = take((size_t)std::max<int64_t>(P.n[0], 1) * (size_t)P.Kr * 8);
     }
-    // unweighted auto-counts evaluate each unshifted unordered pair once; the
-    // Hilbert ranks reuse the sort scratch (free once both sides are built)
+    // unweighted auto-counts evaluate each unshifted unordered pair once; they
+    // need every row-major point's position in the Hilbert order
     P.sym = P.same && !P.obj && (!P.marked || P.tangent);
+    if (P.sym && P.n[0] > 0) {
+        P.o_hpos = take((size_t)P.n[0] * 4);
+        P.o_hrank = take((size_t)P.n[0] * 4);
+    }
     if (P.jack) {
         // (w, tag) rows of each side, and the per-mock subsample tables
         P.o_pk[0] = take((size_t)std::max<int64_t>(P.n[0], 1) * 16);
@@ -504,22 +509,21 @@ static int run(const hp_request* r, int kind) {
         for (int o = 0; o < 2; o++) {
             const Side& S = P.sorted[sd][o];
             if (!S.needed) continue;
-            // with SAME_SAMPLES only side 0 exists; the geometry rows of side 0 describe it
+            // with SAME_SAMPLES only side 0 exists; the geometry rows of side 0 describe it.
+            // Symmetric auto-counts: the Hilbert build (first) leaves each point's
+            // Hilbert position, the row-major gather turns it into the ranks.
+            const bool rank = P.sym && sd == 0 && P.n[0] > 0;
+            uint32_t* hpos = rank ? reinterpret_cast<uint32_t*>(ws + P.o_hpos) : nullptr;
+            uint32_t* hrank = rank ? reinterpret_cast<uint32_t*>(ws + P.o_hrank) : nullptr;
+            static_assert(kHilbert < kRowMajor, "the Hilbert order must be built before the row-major order");
             if (build_sorted_side(dgeom, P.M, sd, (Order)o, X[sd], Y[sd], Z[sd], P.weighted() ? W[sd] : nullptr, P.nw,
                                   P.n[sd], P.total_cells, o == kRowMajor ? P.bits_row : P.bits_hilbert, sb,
-                                  side_ptrs(ws, S), status, s))
+                                  side_ptrs(ws, S), status, s, o == kHilbert ? hpos : nullptr,
+                                  o == kRowMajor ? hpos : nullptr, o == kRowMajor ? hrank : nullptr))
                 return fail("cell build launch failed");
         }
     }
-    const uint32_t* hrank = nullptr;
-    if (P.sym && P.n[0] > 0) {
-        uint32_t* hr = reinterpret_cast<uint32_t*>(ws + P.o_keys[1]);
-        if (launch_hilbert_rank(reinterpret_cast<const uint32_t*>(ws + P.sorted[0][kHilbert].perm),
-                                reinterpret_cast<const uint32_t*>(ws + P.sorted[0][kRowMajor].perm), P.n[0],
-                                reinterpret_cast<uint32_t*>(ws + P.o_keys[0]), hr, s))
-            return fail("rank launch failed");
-        hrank = hr;
-    }
+    const uint32_t* hrank = (P.sym && P.n[0] > 0) ? reinterpret_cast<const uint32_t*>(ws + P.o_hrank) : nullptr;
     if (r->build_events[1] && hipEventRecord(static_cast<hipEvent_t>(r->build_events[1]), s) != hipSuccess)
         return fail("event record failed");
 

@@ -177,7 +177,8 @@ constexpr int kPerWave = kSortTile / kScatterWaves;  // 512 keys, 8 rounds of 64
 __global__ void __launch_bounds__(256) k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                        uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                        int64_t n, int shift, const uint32_t* __restrict__ tile_counts,
-                                                       const uint32_t* __restrict__ digit_totals, int ntiles) {
+                                                       const uint32_t* __restrict__ digit_totals, int ntiles,
+                                                       uint32_t* __restrict__ inv) {
     __shared__ uint32_t skey[kSortTile];
     __shared__ uint32_t sval[kSortTile];
     __shared__ uint32_t wcnt[kScatterWaves][kRadixDigits];  // per-wave digit counts, then offsets
@@ -281,6 +282,7 @@ __global__ void __launch_bounds__(256) k_radix_scatter(const uint32_t* __restric
         const uint32_t pos = gbase[d] + (uint32_t)i - toff[d];
         kout[pos] = key;
         vout[pos] = sval[i];
+        if (inv) inv[sval[i]] = pos;  // last pass: the sorted position of each input point
     }
 }
 
@@ -288,10 +290,11 @@ __global__ void __launch_bounds__(256) k_radix_scatter(const uint32_t* __restric
 __global__ void k_gather(const uint32_t* __restrict__ perm, int64_t n, const double* __restrict__ x,
                          const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ w,
                          int nw, double* __restrict__ ox, double* __restrict__ oy, double* __restrict__ oz,
-                         double* __restrict__ ow) {
+                         double* __restrict__ ow, const uint32_t* __restrict__ hpos, uint32_t* __restrict__ hrank) {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q >= n) return;
     const int64_t p = perm[q];
+    if (hrank) hrank[q] = hpos[p];
     ox[q] = x[p];
     oy[q] = y[p];
     oz[q] = z[p];
@@ -312,32 +315,10 @@ __global__ void k_cell_start(const uint32_t* __restrict__ keys, int64_t n, int64
 
 static inline int check(const char*) { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
-// Position in the Hilbert (lane) order of every point of the row-major
-// (stream) order of the same sample: hpos = inverse of the Hilbert
-// permutation, hrank[j] = hpos[row_perm[j]].
-__global__ void k_inverse_perm(const uint32_t* __restrict__ perm, int64_t n, uint32_t* __restrict__ inv) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) inv[perm[p]] = (uint32_t)p;
-}
-
-__global__ void k_compose_perm(const uint32_t* __restrict__ inv, const uint32_t* __restrict__ perm, int64_t n,
-                               uint32_t* __restrict__ out) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) out[j] = inv[perm[j]];
-}
-
-int launch_hilbert_rank(const uint32_t* hil_perm, const uint32_t* row_perm, int64_t n, uint32_t* hpos,
-                        uint32_t* hrank, hipStream_t s) {
-    if (n <= 0) return 0;
-    const unsigned nb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_inverse_perm, dim3(nb), dim3(256), 0, s, hil_perm, n, hpos);
-    hipLaunchKernelGGL(k_compose_perm, dim3(nb), dim3(256), 0, s, hpos, row_perm, n, hrank);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
 int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, const double* x, const double* y,
                       const double* z, const double* w, int nw, int64_t n, int64_t total_cells, int key_bits,
-                      SortBufs sb, SortedSide out, int32_t* status, hipStream_t s) {
+                      SortBufs sb, SortedSide out, int32_t* status, hipStream_t s, uint32_t* hpos_out,
+                      const uint32_t* hpos_in, uint32_t* hrank_out) {
     if (n <= 0) {
         if (order == kRowMajor) return launch_fill_u32(out.cell_start, 0u, total_cells + 1, s);
         return 0;
@@ -358,7 +339,8 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, c
         uint32_t* vdst = (pass == passes - 1) ? out.perm : sb.vals[cur ^ 1];
         hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(256), 0, s, sb.keys[cur],
                            pass == 0 ? (const uint32_t*)nullptr : sb.vals[cur], sb.keys[cur ^ 1], vdst, n,
-                           shift, sb.tile_counts, sb.digit_totals, ntiles);
+                           shift, sb.tile_counts, sb.digit_totals, ntiles,
+                           pass == passes - 1 ? hpos_out : (uint32_t*)nullptr);
         cur ^= 1;
     }
     if (check("sort")) return -1;
@@ -366,7 +348,7 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, c
     // permutation; only the stream side needs sorted copies.
     if (order == kRowMajor)
         hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, s, out.perm, n, x, y, z, w, nw, out.x, out.y, out.z,
-                           out.w);
+                           out.w, hpos_in, hrank_out);
     if (order == kRowMajor) {
         hipLaunchKernelGGL(k_cell_start, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, sb.keys[cur], n,
                            total_cells, out.cell_start);

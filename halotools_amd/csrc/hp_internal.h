@@ -86,12 +86,16 @@ int launch_store_bytes(void* dst, const void* src, size_t nbytes, hipStream_t s)
 int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 int launch_group_mock(const MockGeom* g, int nmocks, int side, int32_t* group_mock,
                       int64_t ngroups_total, hipStream_t s);
-int launch_hilbert_rank(const uint32_t* hil_perm, const uint32_t* row_perm, int64_t n, uint32_t* hpos,
-                        uint32_t* hrank, hipStream_t s);
+// Symmetric auto-counts also need, for every point of the row-major order,
+// its position in the Hilbert order: the Hilbert build's last radix pass
+// writes that position per point (hpos_out), the row-major gather reads it
+// through its permutation (hpos_in -> hrank_out).  Null pointers skip both.
 int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order,
                       const double* x, const double* y, const double* z, const double* w,
                       int nw, int64_t n, int64_t total_cells, int key_bits,
-                      SortBufs sb, SortedSide out, int32_t* status, hipStream_t s);
+                      SortBufs sb, SortedSide out, int32_t* status, hipStream_t s,
+                      uint32_t* hpos_out = nullptr, const uint32_t* hpos_in = nullptr,
+                      uint32_t* hrank_out = nullptr);
 
 // pairs.hip
 struct PairLaunch {
