@@ -146,6 +146,7 @@ struct Plan {
     bool elim = false;
     size_t o_corr = 0, o_rows_tmp = 0;
     size_t o_hpos = 0, o_hrank = 0;  // symmetric auto-counts: Hilbert positions and ranks
+    size_t o_keys_row = 0;           // row-major keys of a side sorted in both orders
     bool sym = false;  // auto-count (unweighted or tangent): symmetric evaluation
     int crep = 1;
     bool tangent = false;
@@ -375,6 +376,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     }
     const int64_t ntiles = (nmax + kSortTile - 1) / kSortTile;
     P.o_keys[0] = take((size_t)nmax * 4);
+    P.o_keys_row = take((size_t)nmax * 4);
     P.o_keys[1] = take((size_t)nmax * 4);
     P.o_vals[0] = take((size_t)nmax * 4);
     P.o_vals[1] = take((size_t)nmax * 4);
@@ -513,13 +515,17 @@ static int run(const hp_request* r, int kind) {
             // Symmetric auto-counts: the Hilbert build (first) leaves each point's
             // Hilbert position, the row-major gather turns it into the ranks.
             const bool rank = P.sym && sd == 0 && P.n[0] > 0;
+            // a side sorted in both orders computes its keys once (Hilbert build first)
+            const bool both = P.sorted[sd][kHilbert].needed && P.sorted[sd][kRowMajor].needed;
+            uint32_t* krow = both ? reinterpret_cast<uint32_t*>(ws + P.o_keys_row) : nullptr;
             uint32_t* hpos = rank ? reinterpret_cast<uint32_t*>(ws + P.o_hpos) : nullptr;
             uint32_t* hrank = rank ? reinterpret_cast<uint32_t*>(ws + P.o_hrank) : nullptr;
             static_assert(kHilbert < kRowMajor, "the Hilbert order must be built before the row-major order");
             if (build_sorted_side(dgeom, P.M, sd, (Order)o, X[sd], Y[sd], Z[sd], P.weighted() ? W[sd] : nullptr, P.nw,
                                   P.n[sd], P.total_cells, o == kRowMajor ? P.bits_row : P.bits_hilbert, sb,
                                   side_ptrs(ws, S), status, s, o == kHilbert ? hpos : nullptr,
-                                  o == kRowMajor ? hpos : nullptr, o == kRowMajor ? hrank : nullptr))
+                                  o == kRowMajor ? hpos : nullptr, o == kRowMajor ? hrank : nullptr,
+                                  o == kHilbert ? krow : nullptr, o == kRowMajor ? krow : nullptr))
                 return fail("cell build launch failed");
         }
     }

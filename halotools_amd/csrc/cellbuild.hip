@@ -94,9 +94,12 @@ __device__ __forceinline__ int cell_index(double v, double cell, int n, bool& ba
     return (int)f;
 }
 
+// keys_row: when the same side is also sorted in row-major order, the
+// Hilbert pass writes those keys too (one read of the coordinates for both)
 __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, int order,
                        const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-                       int64_t n, uint32_t* __restrict__ keys, int32_t* __restrict__ status) {
+                       int64_t n, uint32_t* __restrict__ keys, int32_t* __restrict__ status,
+                       uint32_t* __restrict__ keys_row) {
     const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int64_t m = mock_of_point(geom, nmocks, side, p);
@@ -106,9 +109,11 @@ __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, 
     const int iy = cell_index(y[p], G.cell[1], G.ndiv[1], bad, G.period[1], G.pbc);
     const int iz = cell_index(z[p], G.cell[2], G.ndiv[2], bad, G.period[2], G.pbc);
     uint32_t key;
+    const uint32_t rkey = (uint32_t)(G.cell_off + ((int64_t)ix * G.ndiv[1] + iy) * G.ndiv[2] + iz);
     if (order == kRowMajor) {
-        key = (uint32_t)(G.cell_off + ((int64_t)ix * G.ndiv[1] + iy) * G.ndiv[2] + iz);
+        key = rkey;
     } else {
+        if (keys_row) keys_row[p] = rkey;
         // lane order: a power-of-two grid of 2^mbits cells per dimension, so the
         // Hilbert curve covers the box exactly (no passes through empty padding)
         const int hb = G.mbits;
@@ -318,26 +323,30 @@ static inline int check(const char*) { return hipGetLastError() == hipSuccess ? 
 int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, const double* x, const double* y,
                       const double* z, const double* w, int nw, int64_t n, int64_t total_cells, int key_bits,
                       SortBufs sb, SortedSide out, int32_t* status, hipStream_t s, uint32_t* hpos_out,
-                      const uint32_t* hpos_in, uint32_t* hrank_out) {
+                      const uint32_t* hpos_in, uint32_t* hrank_out, uint32_t* keys_row_out,
+                      const uint32_t* keys_in) {
     if (n <= 0) {
         if (order == kRowMajor) return launch_fill_u32(out.cell_start, 0u, total_cells + 1, s);
         return 0;
     }
     const unsigned nb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_keys, dim3(nb), dim3(256), 0, s, geom, nmocks, side, (int)order, x, y, z, n, sb.keys[0],
-                       status);
+    // keys_in: this order's keys were written by the other order's key pass
+    if (!keys_in)
+        hipLaunchKernelGGL(k_keys, dim3(nb), dim3(256), 0, s, geom, nmocks, side, (int)order, x, y, z, n, sb.keys[0],
+                           status, order == kHilbert ? keys_row_out : (uint32_t*)nullptr);
     const int ntiles = (int)((n + kSortTile - 1) / kSortTile);
     const int passes = key_bits <= 0 ? 1 : (key_bits + kRadixBits - 1) / kRadixBits;
     int cur = 0;
     for (int pass = 0; pass < passes; pass++) {
         const int shift = pass * kRadixBits;
-        hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(256), 0, s, sb.keys[cur], n, shift, sb.tile_counts,
+        const uint32_t* kin = (pass == 0 && keys_in) ? keys_in : sb.keys[cur];
+        hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(256), 0, s, kin, n, shift, sb.tile_counts,
                            ntiles);
         hipLaunchKernelGGL(k_radix_scan, dim3(kRadixDigits), dim3(256), 0, s, sb.tile_counts, ntiles,
                            sb.digit_totals);
         // the last pass writes the permutation straight into out.perm
         uint32_t* vdst = (pass == passes - 1) ? out.perm : sb.vals[cur ^ 1];
-        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(256), 0, s, sb.keys[cur],
+        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(256), 0, s, kin,
                            pass == 0 ? (const uint32_t*)nullptr : sb.vals[cur], sb.keys[cur ^ 1], vdst, n,
                            shift, sb.tile_counts, sb.digit_totals, ntiles,
                            pass == passes - 1 ? hpos_out : (uint32_t*)nullptr);

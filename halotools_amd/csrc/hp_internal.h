@@ -89,13 +89,17 @@ int launch_group_mock(const MockGeom* g, int nmocks, int side, int32_t* group_mo
 // Symmetric auto-counts also need, for every point of the row-major order,
 // its position in the Hilbert order: the Hilbert build's last radix pass
 // writes that position per point (hpos_out), the row-major gather reads it
-// through its permutation (hpos_in -> hrank_out).  Null pointers skip both.
+// through its permutation (hpos_in -> hrank_out).  When one side is sorted in
+// both orders, the Hilbert build also writes the row-major keys
+// (keys_row_out) and the row-major build starts from them (keys_in).  Null
+// pointers skip each of these.
 int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order,
                       const double* x, const double* y, const double* z, const double* w,
                       int nw, int64_t n, int64_t total_cells, int key_bits,
                       SortBufs sb, SortedSide out, int32_t* status, hipStream_t s,
                       uint32_t* hpos_out = nullptr, const uint32_t* hpos_in = nullptr,
-                      uint32_t* hrank_out = nullptr);
+                      uint32_t* hrank_out = nullptr, uint32_t* keys_row_out = nullptr,
+                      const uint32_t* keys_in = nullptr);
 
 // pairs.hip
 struct PairLaunch {
