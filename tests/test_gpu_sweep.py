@@ -21,7 +21,7 @@ from oracle import mesh as ORC
 pytestmark = pytest.mark.gpu
 
 # cases per counter family; HALOTOOLS_AMD_SWEEP_SEEDS=N widens the sweep
-# (N count cases, N/2 marked, N/3 jackknife)
+# (N count cases, N/2 marked, N/3 jackknife, N/8 medium-size)
 NSEEDS = int(os.environ.get("HALOTOOLS_AMD_SWEEP_SEEDS", "24"))
 
 
@@ -133,3 +133,44 @@ def test_jackknife_sweep(mo, seed):
             got = mo.npairs_jackknife_xy_z(a, b, rb, pi, ta, tb, ns, period=period)
             ref = ORC.npairs_jackknife(a, b, rb, ta, tb, ns, pi_bins=pi, period=period, num_threads=4)
             assert np.array_equal(got, ref), "jackknife xy_z"
+
+
+def _medium_case(seed):
+    # 20k-200k points (uniform or in tight clumps), small radii: many lane
+    # groups per cell, dense histograms, several groups per count replica
+    rng = np.random.RandomState(7000 + seed)
+    pbc = bool(rng.randint(0, 4))
+    box = np.array([1.0, 1.0, 1.0]) if rng.randint(0, 2) else rng.uniform(0.7, 1.3, 3)
+    n1 = int(rng.choice([20000, 65536, 100001, 200000]))
+    n2 = int(rng.choice([30000, 150000]))
+    clumped = bool(rng.randint(0, 2))
+
+    def pts(n):
+        if not clumped:
+            return rng.uniform(0, 1, (n, 3)) * box
+        ncl = n // 40
+        cen = rng.uniform(0, 1, (ncl, 3)) * box
+        return np.mod(cen[rng.randint(0, ncl, n)] + rng.normal(0, 0.004, (n, 3)) * box, box)
+
+    s1, s2 = pts(n1), pts(n2)
+    rmax = rng.uniform(0.005, 0.04) * box.min()
+    rb = np.sort(rng.uniform(0, rmax, int(rng.choice([2, 8, 20, 70]))))
+    rb[-1] = rmax
+    pi = np.sort(rng.uniform(0, rmax, int(rng.choice([2, 5, 12]))))
+    pi[-1] = rmax
+    w1, w2 = rng.uniform(0, 1, n1), rng.uniform(0, 1, n2)
+    return s1, s2, rb, pi, (box if pbc else None), w1, w2
+
+
+@pytest.mark.parametrize("seed", range(max(2, NSEEDS // 8)))
+def test_medium_sweep(mo, seed):
+    s1, s2, rb, pi, period, w1, w2 = _medium_case(seed)
+    for a, b, wa, wb in ((s1, s2, w1, w2), (s1, s1, w1, w1)):
+        assert np.array_equal(mo.npairs_3d(a, b, rb, period=period),
+                              ORC.npairs_3d(a, b, rb, period=period, num_threads=8)), "npairs_3d"
+        assert np.array_equal(mo.npairs_xy_z(a, b, rb, pi, period=period),
+                              ORC.npairs_xy_z(a, b, rb, pi, period=period, num_threads=8)), "npairs_xy_z"
+        got = mo.marked_npairs_3d(a, b, rb, 1, period=period, weights1=wa, weights2=wb)
+        ref = ORC.marked_npairs_3d(a, b, rb, period=period, weights1=wa, weights2=wb, weight_func_id=1,
+                                   num_threads=8)
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-300)
