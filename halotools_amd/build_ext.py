@@ -3,6 +3,7 @@
 The library is loaded through ctypes (halotools_amd/_engine.py); it is plain
 HIP C++ with an extern "C" ABI (include/halopair.h) and no PyTorch types.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -10,7 +11,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_lib", "libhalopair.so")
-SOURCES = ["cellbuild.hip", "pairs.hip", "reduce.hip", "capi.hip"]
+# the pair kernel's variants are instantiated one mode per translation unit
+# (pairs_mode_*.hip) so that they compile in parallel
+SOURCES = ["cellbuild.hip", "pairs.hip", "reduce.hip", "capi.hip"] + [
+    "pairs_mode_%s.hip" % m for m in ("count3d", "proj", "xyz", "obj", "wxyz", "jack", "jackxy", "wsum", "jac", "tan")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -19,7 +23,6 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # bit-exact FP64: the reference engines evaluate dx*dx + dy*dy + dz*dz
     # without fused multiply-add, so no contraction anywhere
     "-ffp-contract=off",
@@ -29,10 +32,25 @@ FLAGS = [
 ]
 
 
-def build(verbose=False, extra=None, out=None):
+def build(verbose=False, extra=None, out=None, jobs=None):
+    """Compile every source to an object (in parallel), then link the library."""
     out = out or OUT
-    os.makedirs(os.path.dirname(out), exist_ok=True)
-    cmd = [HIPCC] + FLAGS + (extra or []) + [os.path.join(SRC, s) for s in SOURCES] + ["-o", out + ".tmp"]
+    objdir = out + ".obj"
+    os.makedirs(objdir, exist_ok=True)
+    flags = FLAGS + (extra or [])
+
+    def compile_one(src):
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        cmd = [HIPCC] + flags + ["-c", os.path.join(SRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        return obj
+
+    jobs = jobs or min(len(SOURCES), max(1, min(os.cpu_count() or 1, 16)))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+        objs = list(pool.map(compile_one, SOURCES))
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", out + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
