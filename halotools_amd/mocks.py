@@ -82,3 +82,21 @@ def config_points(cfg, rng=None):
         p = uniform_box(1000000, 1000.0, rng)
         return p, p, 1000.0
     raise ValueError("configurations 1-4 only")
+
+
+def many_mocks(rng=None, nmocks=64, nmin=5e4, nmax=2e6, L=1000.0):
+    """The batched model-fitting workload, seeded with default_rng([43, 5]).
+
+    `nmocks` realisations of N_m points, N_m log-uniform in [nmin, nmax],
+    uniform in a periodic cube of side L; halo masses log10 M ~ U[11, 15] per
+    point; one Zheng07 logMmin ~ U[11.5, 12.5] per mock; loss coefficients
+    c_k ~ N(0, 1), one per edge of ``config_rbins(5)``.  Returns
+    (sizes, points list, log-mass list, logMmin array, c array, L).
+    """
+    rng = rng if rng is not None else np.random.default_rng([43, 5])
+    sizes = np.round(np.exp(rng.uniform(np.log(nmin), np.log(nmax), nmocks))).astype(int)
+    pts = [rng.uniform(0, L, (n, 3)) for n in sizes]
+    logm = [rng.uniform(11, 15, n) for n in sizes]
+    logMmin = rng.uniform(11.5, 12.5, nmocks)
+    c = rng.normal(size=len(config_rbins(5)))
+    return sizes, pts, logm, logMmin, c, L

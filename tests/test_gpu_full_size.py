@@ -1,10 +1,14 @@
 """GPU parity at the benchmark's full sizes.
 
 The ~1M-point NFW-clustered box (npairs_3d, per-object rows, jackknife table),
-2M uniform points (npairs_xy_z with 16 x 41 bins and npairs_projected), and
-1M uniform halos with Zheng07 weights (product-marked sums): against the
-oracle run on all host threads (bit-exact counts; marked sums within 1e-12
-of the oracle's compensated sum), and
+2M uniform points (npairs_xy_z with 16 x 41 bins and npairs_projected),
+1M uniform halos with Zheng07 weights (product-marked sums and their gradient
+with respect to (logMmin, sigma_logM)), and the 64-mock batch of 50k-2M
+points (counts, marked sums and dloss/dlogMmin of every mock from one
+captured step): against the oracle run on the host threads (bit-exact counts;
+marked sums within 1e-12 of the oracle's compensated sum; gradients against
+central finite differences within 1e-6 and against compensated oracle sums of
+the pair terms t_i w_j + w_i t_j), and
 through identities that hold at any size:
   * the projected counts equal the pi = pi_max column of the xy_z counts;
   * per-object rows add up to npairs_3d;
@@ -100,3 +104,146 @@ def test_uniform_1M_marked_product(gpu):
     finally:
         ORC.set_compensated(False)
     np.testing.assert_allclose(got, ref, rtol=1e-12)
+
+
+def _dlogmmin_weights(logm, logMmin, sigma):
+    """Zheng07 central weights and their derivative with respect to logMmin (NumPy, FP64)."""
+    from scipy.special import erf
+    u = (logm - logMmin) / sigma
+    return 0.5 * (1.0 + erf(u)), -np.exp(-u * u) / (np.sqrt(np.pi) * sigma)
+
+
+def _oracle_marked(p, rb, L, w1, w2):
+    """Compensated oracle sums of w1_i w2_j over ordered pairs of the auto-count of p."""
+    ORC.set_compensated(True)
+    try:
+        return ORC.marked_npairs_3d(p, p, rb, period=L, weights1=w1, weights2=w2, weight_func_id=1,
+                                    num_threads=THREADS)
+    finally:
+        ORC.set_compensated(False)
+
+
+def test_uniform_1M_marked_gradient(gpu):
+    """C4 at full size: d(sum_k c_k W_k)/d(logMmin, sigma_logM) of 1M uniform halos.
+
+    Checked against central finite differences of the GPU forward (1e-6) and
+    against the oracle: dW_k/dtheta = sum over ordered pairs within r_k of
+    t_i w_j + w_i t_j (t = dw/dtheta), i.e. two compensated product-marked
+    oracle sums with (t, w) and (w, t) as the weights.
+    """
+    import torch
+    from halotools_amd import mocks
+    from halotools_amd.autodiff import marked_npairs_3d_product
+    rng = np.random.default_rng([43, 4])
+    p = mocks.uniform_box(1000000, 1000.0, rng)
+    logm_h = rng.uniform(11, 15, len(p))
+    logm = torch.tensor(logm_h, device=gpu)
+    rb = mocks.config_rbins(4)
+    c = np.random.default_rng(7).normal(size=len(rb))
+    ct = torch.tensor(c, device=gpu)
+    t = torch.tensor(p, device=gpu)
+
+    def loss(theta):
+        w = mocks.zheng07_ncen(logm, theta[0], theta[1])
+        return (marked_npairs_3d_product(t, t, rb, w, w, period=1000.0) * ct).sum()
+
+    theta = torch.tensor([12.0, 0.3], dtype=torch.float64, device=gpu, requires_grad=True)
+    loss(theta).backward()
+    grad = theta.grad.cpu().numpy()
+    for a, h in ((0, 1e-4), (1, 1e-5)):
+        tp = theta.detach().clone()
+        tp[a] += h
+        tm = theta.detach().clone()
+        tm[a] -= h
+        fd = (loss(tp).item() - loss(tm).item()) / (2 * h)
+        assert abs(fd - grad[a]) <= 1e-6 * abs(fd), (a, fd, grad[a])
+
+    w, t_min = _dlogmmin_weights(logm_h, 12.0, 0.3)
+    u = (logm_h - 12.0) / 0.3
+    t_sig = -np.exp(-u * u) * u / (np.sqrt(np.pi) * 0.3)   # dw / dsigma_logM
+    for a, tw in ((0, t_min), (1, t_sig)):
+        dW = _oracle_marked(p, rb, 1000.0, tw, w) + _oracle_marked(p, rb, 1000.0, w, tw)
+        ref = float(np.dot(c, dW))
+        assert abs(ref - grad[a]) <= 1e-9 * abs(ref), (a, ref, grad[a])
+
+
+@pytest.fixture(scope="module")
+def c5(gpu):
+    """The 64-mock batch of the benchmark (50k-2M points each), tuned, one captured step replayed."""
+    import torch
+    from halotools_amd import autotune, batch, mocks
+    sizes, pts, logm, logMmin, c, L = mocks.many_mocks()
+    rb = mocks.config_rbins(5)
+    tp = [torch.tensor(x, device=gpu) for x in pts]
+    b = batch.MockBatch(tp, L, rb, device=gpu)
+    b.tune(autotune.TANGENT, reps=1)
+    lm = torch.tensor(np.concatenate(logm), device=gpu)
+    step = batch.HODBatchStep(b, lm, torch.tensor(logMmin, device=gpu), 0.3, torch.tensor(c, device=gpu))
+    step.capture()
+    out = [x.clone() for x in step.replay()]
+    tangents = step.res.tangents.clone()
+    wt = step.wt.cpu().numpy()
+    torch.cuda.synchronize()
+    assert step.status() == 0
+    return dict(sizes=sizes, pts=pts, logMmin=logMmin, c=c, L=L, rb=rb, tp=tp, lm=lm, batch=b, step=step,
+                out=out, tangents=tangents, wt=wt)
+
+
+def test_c5_counts_equal_single_calls(c5):
+    from halotools_amd.mock_observables import npairs_3d
+    counts = c5["out"][0].cpu().numpy()
+    for m, t in enumerate(c5["tp"]):
+        assert np.array_equal(counts[m], npairs_3d(t, t, c5["rb"], period=c5["L"]).cpu().numpy()), m
+
+
+def test_c5_smallest_and_largest_mock_exact_vs_oracle(c5):
+    counts = c5["out"][0].cpu().numpy()
+    for m in (int(np.argmin(c5["sizes"])), int(np.argmax(c5["sizes"]))):
+        p = c5["pts"][m]
+        ref = ORC.npairs_3d(p, p, c5["rb"], period=c5["L"], num_threads=THREADS)
+        assert np.array_equal(counts[m], ref), m
+
+
+def test_c5_largest_mock_sums_and_tangents_vs_oracle(c5):
+    """W and dW/dlogMmin of the largest (~1.9M-point) mock against compensated oracle sums."""
+    m = int(np.argmax(c5["sizes"]))
+    offs = c5["batch"].offsets
+    p = c5["pts"][m]
+    w, t = c5["wt"][offs[m]:offs[m + 1], 0], c5["wt"][offs[m]:offs[m + 1], 1]
+    W = c5["out"][1][m].cpu().numpy()
+    np.testing.assert_allclose(W, _oracle_marked(p, c5["rb"], c5["L"], w, w), rtol=1e-12)
+    dW = _oracle_marked(p, c5["rb"], c5["L"], t, w) + _oracle_marked(p, c5["rb"], c5["L"], w, t)
+    np.testing.assert_allclose(c5["tangents"][m].cpu().numpy(), dW, rtol=1e-12)
+    ref = float(np.dot(c5["c"], dW))
+    got = c5["out"][3][m].item()
+    assert abs(got - ref) <= 1e-10 * abs(ref), (got, ref)
+
+
+def test_c5_gradient_vs_finite_differences(c5):
+    step = c5["step"]
+    grad = c5["out"][3].clone()
+    h = 1e-4
+    base = step.logMmin.clone()
+    try:
+        step.logMmin.copy_(base + h)
+        lp = step.run()[2].clone()
+        step.logMmin.copy_(base - h)
+        lm = step.run()[2].clone()
+    finally:
+        step.logMmin.copy_(base)
+    fd = (lp - lm) / (2 * h)
+    rel = ((fd - grad).abs() / fd.abs()).max().item()
+    assert rel <= 1e-6, rel
+
+
+def test_c5_reverse_mode_step_agrees(c5):
+    """The reverse-mode step (count pass, Jacobian rows, backward contraction) on the same batch."""
+    import torch
+    from halotools_amd import batch
+    st = batch.HODBatchStep(c5["batch"], c5["lm"], torch.tensor(c5["logMmin"], device=c5["lm"].device), 0.3,
+                            torch.tensor(c5["c"], device=c5["lm"].device), mode="reverse")
+    counts, W, loss, grad = [x.clone() for x in st.run()]
+    assert st.status() == 0
+    assert torch.equal(counts, c5["out"][0])
+    np.testing.assert_allclose(W.cpu().numpy(), c5["out"][1].cpu().numpy(), rtol=1e-12)
+    np.testing.assert_allclose(grad.cpu().numpy(), c5["out"][3].cpu().numpy(), rtol=1e-10)
