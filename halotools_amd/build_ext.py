@@ -58,5 +58,28 @@ def build(verbose=False, extra=None, out=None, jobs=None):
     return out
 
 
+SAN_OUT = os.path.join(HERE, "_lib", "san", "libhalopair_san.so")
+# host code only: the device code is built as usual
+SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-g"]
+
+
+def build_host_sanitized(out=None):
+    """The library with its host code (request planning, bin tables, the C ABI)
+    under AddressSanitizer and UndefinedBehaviorSanitizer, for CPU tests.
+
+    Load it into a process started with ROCm clang's ASan runtime preloaded
+    (``sanitizer_runtime()``) and ``HALOPAIR_LIB`` pointing at it.
+    """
+    return build(extra=SAN_FLAGS, out=out or SAN_OUT)
+
+
+def sanitizer_runtime():
+    """Path of the ASan runtime of ROCm's clang (shared; includes the UBSan handlers)."""
+    import glob
+    found = sorted(glob.glob("/opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so"))
+    return found[-1] if found else None
+
+
 if __name__ == "__main__":
     build(verbose=True, extra=sys.argv[1:])

@@ -64,6 +64,8 @@ def load(fast=False):
     if name in _LIBS:
         return _LIBS[name]
     path = os.path.join(_BUILD, name)
+    if not fast and os.environ.get("HP_ORACLE_LIB"):
+        path = os.environ["HP_ORACLE_LIB"]  # e.g. the sanitizer build (make -C oracle san)
     if not os.path.exists(path):
         build()
     lib = ctypes.CDLL(path)

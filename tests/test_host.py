@@ -326,3 +326,64 @@ def test_tensor_xyz_rows_detects_one_block_of_rows():
     assert D.tensor_xyz_rows(P.columns(torch.zeros((10, 4), dtype=torch.float64))) is None
     assert D.tensor_xyz_rows(P.columns(t.t().contiguous().t())) is None
     assert D.tensor_xyz_rows(P.columns(torch.zeros((0, 3), dtype=torch.float64))) is None
+
+
+def test_request_planning_sweep():
+    """hp_workspace_size over seeded random requests: every kind and flag
+    combination, several mocks of uneven size (some empty), periodic or not,
+    cell hints, edge counts up to the table limits, and invalid values.  Each
+    request is planned twice (the size is a pure function of the request);
+    an invalid one returns 0 with a message.  The entry points themselves
+    reject the same requests without a workspace, before touching any device
+    pointer.  (Run under AddressSanitizer / UBSan by test_sanitizers.py.)"""
+    lib = E.library()
+    rng = np.random.default_rng(2024)
+    entries = [lib.hp_npairs_3d, lib.hp_npairs_projected, lib.hp_npairs_xy_z, lib.hp_marked_npairs_3d,
+               lib.hp_npairs_per_object_3d, lib.hp_marked_npairs_xy_z, lib.hp_npairs_jackknife_3d,
+               lib.hp_npairs_jackknife_xy_z]
+    planned = rejected = 0
+    for it in range(400):
+        kind = int(rng.integers(0, 8))
+        M = int(rng.integers(1, 5))
+        mocks = (E.HpMock * M)()
+        for m in range(M):
+            n = int(rng.choice([0, 1, 63, 64, 65, 1000, int(rng.integers(0, 200000))]))
+            mocks[m].n1, mocks[m].n2 = n, int(rng.integers(0, 5000))
+            L = float(rng.uniform(50, 2000))
+            mocks[m].period[:] = [L, L * rng.uniform(0.5, 2), L * rng.uniform(0.5, 2)]
+            mocks[m].pbc = int(rng.integers(0, 2))
+            if rng.random() < 0.3:
+                mocks[m].cell[:] = [float(rng.uniform(-1, 60))] * 3
+        nr = int(rng.choice([1, 2, 3, 20, 64, 65, 128, 129, 511, 512, 513]))
+        np_ = int(rng.choice([0, 1, 2, 41, 100, 512, 513]))
+        r2 = np.sort(rng.uniform(0.0, 15.0, nr)) ** 2
+        if rng.random() < 0.1:
+            r2 = r2[::-1].copy()        # non-monotone edges are accepted (suffix minimum)
+        if rng.random() < 0.05:
+            r2[0] = np.nan
+        p2 = np.sort(rng.uniform(0.0, 15.0, max(np_, 1))) ** 2
+        req = E.HpRequest()
+        req.kind = kind
+        req.flags = int(rng.integers(0, 64))
+        req.nmocks = M
+        req.wfunc = int(rng.integers(0, 19))
+        req.nw = int(rng.integers(0, 7))
+        req.nr = nr
+        req.np = np_
+        req.n_samples = int(rng.integers(0, 300))
+        req.tile_waves = int(rng.choice([0, 1, 2, 3, 4, 8]))
+        req.mocks = ctypes.cast(mocks, ctypes.POINTER(E.HpMock))
+        req.r2 = r2.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        req.p2 = p2.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        size = lib.hp_workspace_size(ctypes.byref(req))
+        assert size == lib.hp_workspace_size(ctypes.byref(req))
+        if size == 0:
+            rejected += 1
+            assert lib.hp_last_error().decode()
+        else:
+            planned += 1
+            assert size % 256 == 0
+        # no workspace: rejected before any device work
+        assert entries[kind](ctypes.byref(req)) < 0
+        assert lib.hp_last_error().decode()
+    assert planned > 50 and rejected > 50, (planned, rejected)
