@@ -34,6 +34,7 @@ EXPORTED_SYMBOLS = (
     "hp_npairs_projected", "hp_npairs_xy_z", "hp_marked_npairs_3d", "hp_npairs_per_object_3d",
     "hp_marked_npairs_xy_z", "hp_npairs_jackknife_3d", "hp_npairs_jackknife_xy_z", "hp_marked_backward",
     "hp_segment_sum_scratch", "hp_segment_sum", "hp_debug_bin_lookup", "hp_debug_hilbert",
+    "hp_npairs_3d_noalloc",
 )
 
 
@@ -107,7 +108,7 @@ def library():
     lib.hp_workspace_size.restype = ctypes.c_size_t
     for name in ("hp_npairs_3d", "hp_npairs_projected", "hp_npairs_xy_z", "hp_marked_npairs_3d",
                  "hp_npairs_per_object_3d", "hp_marked_npairs_xy_z", "hp_npairs_jackknife_3d",
-                 "hp_npairs_jackknife_xy_z"):
+                 "hp_npairs_jackknife_xy_z", "hp_npairs_3d_noalloc"):
         fn = getattr(lib, name)
         fn.argtypes = [ctypes.POINTER(HpRequest)]
         fn.restype = ctypes.c_int

@@ -11,6 +11,7 @@
 #include <cstring>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -651,6 +652,38 @@ static int run(const hp_request* r, int kind) {
     return 0;
 }
 
+// ------------------------------------------------------------ no-allocation variant
+// hp_npairs_3d_noalloc runs on a fixed scratch array that every GPU the
+// library runs on holds from module load until the process ends, instead of
+// a caller-allocated workspace.  Host code must never pass the array's name
+// to a kernel (it is a host-side stand-in there); each device's own address
+// of it comes from hipGetSymbolAddress, called once per device with that
+// device current, and only that address is used.  Calls that overlap on one
+// GPU share the array and race.
+static_assert(HP_NOALLOC_SCRATCH_BYTES % 256 == 0, "scratch size must keep 256-byte alignment");
+static __device__ __align__(256) unsigned char scratch[HP_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_of_current_device(void** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return fail("hp_npairs_3d_noalloc: no current HIP device");
+    if (dev >= kMaxDevices) return fail("hp_npairs_3d_noalloc: device index beyond the library's table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(std::string("hp_npairs_3d_noalloc: hipGetSymbolAddress(scratch) failed: ") +
+                        hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 extern "C" {
 
 const char* hp_last_error(void) { return g_err.c_str(); }
@@ -665,6 +698,25 @@ size_t hp_workspace_size(const hp_request* req) {
 }
 
 int hp_npairs_3d(const hp_request* req) { return run(req, HP_NPAIRS_3D); }
+
+int hp_npairs_3d_noalloc(const hp_request* req) {
+    if (!req) return fail("null request");
+    if (req->workspace || req->workspace_bytes)
+        return fail("hp_npairs_3d_noalloc takes no workspace (workspace must be NULL and workspace_bytes 0)");
+    Plan P;
+    if (make_plan(req, HP_NPAIRS_3D, P)) return -1;
+    if (P.total > (size_t)HP_NOALLOC_SCRATCH_BYTES)
+        return fail("hp_npairs_3d_noalloc: the request needs " + std::to_string(P.total) +
+                    " bytes of scratch, more than the library's fixed " +
+                    std::to_string((size_t)HP_NOALLOC_SCRATCH_BYTES) +
+                    " bytes (use hp_npairs_3d with a workspace of hp_workspace_size() bytes)");
+    void* ptr = nullptr;
+    if (scratch_of_current_device(&ptr)) return -1;
+    hp_request r = *req;
+    r.workspace = ptr;
+    r.workspace_bytes = (size_t)HP_NOALLOC_SCRATCH_BYTES;
+    return run(&r, HP_NPAIRS_3D);
+}
 int hp_npairs_projected(const hp_request* req) { return run(req, HP_NPAIRS_PROJECTED); }
 int hp_npairs_xy_z(const hp_request* req) { return run(req, HP_NPAIRS_XY_Z); }
 int hp_marked_npairs_3d(const hp_request* req) { return run(req, HP_MARKED_NPAIRS_3D); }

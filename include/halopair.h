@@ -43,6 +43,7 @@ extern "C" {
 
 #define HP_ABI_VERSION 1
 #define HP_MAX_BINS 512 /* per axis */
+#define HP_NOALLOC_SCRATCH_BYTES (64u << 20) /* fixed scratch of hp_npairs_3d_noalloc, per GPU */
 
 /* Which reference engine a request replaces. */
 enum hp_kind {
@@ -137,6 +138,18 @@ size_t hp_workspace_size(const hp_request* req);
  * (PC/cpairs/npairs_3d_engine.pyx:17-184) and the RectangularDoubleMesh build
  * (PC/rectangular_mesh.py:228-374) that feeds it. */
 int hp_npairs_3d(const hp_request* req);
+
+/* npairs_3d without a workspace: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the same request and results
+ * as hp_npairs_3d, with `workspace` NULL and `workspace_bytes` 0; the scratch is
+ * a fixed HP_NOALLOC_SCRATCH_BYTES (64 MiB) array that every GPU the library
+ * runs on holds from module load until the process ends.  A request fits when
+ * hp_workspace_size(req) <= HP_NOALLOC_SCRATCH_BYTES (an auto-count of about
+ * 1.1M points with 20 bins); a larger one returns an error.  The current HIP
+ * device's array is used (the stream must belong to that device), and it keeps
+ * the last call's contents: calls that overlap on one GPU race. */
+int hp_npairs_3d_noalloc(const hp_request* req);
 
 /* npairs_projected: counts[m][k] of pairs with dxy^2 <= r2[k] and
  * dz^2 <= p2[0].  Replaces PC/cpairs/npairs_projected_engine.pyx:17-192. */

@@ -387,3 +387,19 @@ def test_request_planning_sweep():
         assert entries[kind](ctypes.byref(req)) < 0
         assert lib.hp_last_error().decode()
     assert planned > 50 and rejected > 50, (planned, rejected)
+
+
+def test_noalloc_entry_rejects_before_device_work():
+    # argument and size errors of hp_npairs_3d_noalloc come from host planning alone
+    lib = E.library()
+    req, keep = _request(E.HP_NPAIRS_3D)
+    req.workspace = 16
+    req.workspace_bytes = 16
+    assert lib.hp_npairs_3d_noalloc(ctypes.byref(req)) < 0
+    assert "takes no workspace" in lib.hp_last_error().decode()
+    req, keep = _request(E.HP_NPAIRS_3D)
+    keep[0].n1 = keep[0].n2 = 5000000
+    assert lib.hp_workspace_size(ctypes.byref(req)) > (64 << 20)
+    assert lib.hp_npairs_3d_noalloc(ctypes.byref(req)) < 0
+    assert "more than the library's fixed 67108864 bytes" in lib.hp_last_error().decode()
+    assert lib.hp_npairs_3d_noalloc(None) < 0
