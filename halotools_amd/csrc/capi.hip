@@ -92,18 +92,18 @@ static Table make_table(const std::vector<double>& e_in, int K) {
         if (b <= 0) return 0.0;
         return from_hiword((uint32_t)((int64_t)(b + base) << shift));
     };
+    // e[0..K-1] is ascending: the edges below a bucket and inside it are
+    // found by binary search (O(nb log K) for any number of edges)
     int multi = 0;
+    const double* e0 = T.e.data();
     for (int b = 0; b < nb; b++) {
         const double l = lo(b);
         const double h = (b + 1 < nb) ? lo(b + 1) : INFINITY;
-        int below = 0, inside = 0;
-        for (int k = 0; k < K; k++) {
-            if (T.e[k] < l) below++;
-            else if (T.e[k] < h || (b + 1 == nb)) inside++;
-        }
+        const int below = (int)(std::lower_bound(e0, e0 + K, l) - e0);
+        const int upto = (b + 1 == nb) ? K : (int)(std::lower_bound(e0, e0 + K, h) - e0);
         T.t[(size_t)b].k = (uint32_t)below;
         T.t[(size_t)b].edge = T.e[(size_t)below];
-        if (inside > 1) multi = 1;
+        if (upto - below > 1) multi = 1;
     }
     T.spec.multi = multi;
     return T;
@@ -158,6 +158,7 @@ struct Plan {
     size_t o_pk[2] = {0, 0}, o_jacc = 0;
     bool weighted() const { return marked || jack; }
     size_t o_ptan = 0;
+    size_t o_queue = 0;
     size_t total = 0;
     std::vector<unsigned char> consts;  // bytes [0, o_gm[0])
 };
@@ -197,6 +198,9 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
         if (r->n_samples < 1) return fail("n_samples must be >= 1");
         if ((r->w1 || r->w2) && r->nw != 1) return fail("jackknife weights are one per point (nw = 1)");
         P.nsamp = r->n_samples;
+        // LDS rows per wave for the tags a group meets; the pair launch caps
+        // them further to ~24 KB of table per wave
+        P.jslots = std::min(kJackSlots, P.nsamp);
     }
     if (r->tile_waves != 0 && r->tile_waves != 1 && r->tile_waves != 2 && r->tile_waves != 4 && r->tile_waves != 8)
         return fail("tile_waves must be 0, 1, 2, 4 or 8");
@@ -220,11 +224,9 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
         if (P.Kp < 1 || P.Kp > HP_MAX_BINS || !r->p2) return fail("pi bins out of range");
         if (kind == HP_NPAIRS_PROJECTED && P.Kp != 1) return fail("npairs_projected takes one pi_max");
     }
-    if (kind == HP_NPAIRS_XY_Z && P.Kr * P.Kp > 16384) return fail("rp_bins x pi_bins exceeds 16384 cells");
-    if (P.marked && P.Kr > 128) return fail("marked counts support at most 128 bins");
-    // one FP64 (rp, pi) histogram per wavefront in LDS, with a trash row and column
-    if (P.mxyz && (P.Kr + 1) * (P.Kp + 1) > 5800)
-        return fail("marked_npairs_xy_z supports at most 5800 (len(rp_bins)+1)*(len(pi_bins)+1) cells");
+    // any number of radial / rp bins: histograms that do not fit in one
+    // launch's LDS are counted over several launches of bin ranges
+    if ((int64_t)P.Kr * std::max(P.Kp, 1) > ((int64_t)1 << 26)) return fail("more than 2^26 histogram cells");
 
     std::vector<double> er = suffix_min(r->r2, P.Kr);
     P.tr = make_table(er, P.Kr);
@@ -393,6 +395,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     if (P.sums) P.o_psum = take((size_t)gA * (size_t)P.nb_count * 8);
     if (P.tangent) P.o_ptan = take((size_t)gA * (size_t)P.Kr * 8);
     P.o_pevals = take((size_t)gA * 8);
+    P.o_queue = take(64 * 4);  // work-queue counters of the pair launches
     const int64_t red = reduce_scratch_elems(std::max<int64_t>(P.max_groups[0], 1), P.M,
                                              P.nb_count);  // sums (Kr or Kr * Kp) and evals (1)
     P.o_red = take((size_t)red * 8);
@@ -416,10 +419,6 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
         const double tab = (double)P.M * (P.nsamp + 1.0) * (double)P.nb_count * 8.0;
         if (tab > 64.0 * 1024 * 1024 * 1024) return fail("jackknife tables exceed 64 GiB");
         P.o_jacc = take((size_t)tab);
-        // LDS rows per wave for the tags a group meets: ~24 KB of table per wave
-        const int jrow = (P.Kr + 1) * (xyz ? P.Kp + 1 : 1) * 8;
-        P.jslots = std::max(0, std::min({kJackSlots, P.nsamp, 24576 / jrow - 1}));
-        if (jrow > 48 * 1024) return fail("jackknife: (len(rp_bins)+1)*(len(pi_bins)+1) exceeds 6144 cells");
     }
     P.total = align_up(off, 256);
 
@@ -556,6 +555,7 @@ static int run(const hp_request* r, int kind) {
     L.psum = P.sums ? reinterpret_cast<double*>(ws + P.o_psum) : nullptr;
     L.ptan = P.tangent ? reinterpret_cast<double*>(ws + P.o_ptan) : nullptr;
     L.pevals = reinterpret_cast<uint64_t*>(ws + P.o_pevals);
+    L.queue = reinterpret_cast<uint32_t*>(ws + P.o_queue);
     switch (kind) {
         case HP_NPAIRS_3D: L.mode = kModeCount3D; break;
         case HP_NPAIRS_PROJECTED: L.mode = kModeProj; break;
@@ -596,7 +596,9 @@ static int run(const hp_request* r, int kind) {
         return fail("event record failed");
     if (!(P.marked && !P.sums && !P.counts && !P.rows1 && !P.elim)) {
         rc = launch_pairs(L, s);
-        if (rc == -2) return fail("histogram does not fit in LDS (too many bins)");
+        if (rc == -2)
+            return fail("len(pi_bins) too large: one rp row of the (rp, pi) histogram must fit in the LDS of a "
+                        "workgroup");
         if (rc) return fail("pair kernel launch failed");
     }
     if (P.elim) {

@@ -134,6 +134,9 @@ struct PairLaunch {
     double* jacc;
     int nsamp;
     int jslots;            // LDS rows per wave for subsample tags (<= kJackSlots)
+    // persistent launches: one work-queue counter per XCD (8 words, zeroed
+    // by launch_pairs before every launch)
+    uint32_t* queue;
 };
 enum Mode : int { kModeCount3D = 0, kModeProj = 1, kModeXYZ = 2, kModeWSum = 3, kModeJac = 4, kModeTan = 5,
                   kModeObj = 6, kModeWXYZ = 7, kModeJack = 8, kModeJackXY = 9 };

@@ -42,7 +42,7 @@ extern "C" {
 #endif
 
 #define HP_ABI_VERSION 1
-#define HP_MAX_BINS 512 /* per axis */
+#define HP_MAX_BINS (1 << 20) /* edges per axis (and at most 2^26 histogram cells) */
 #define HP_NOALLOC_SCRATCH_BYTES (64u << 20) /* fixed scratch of hp_npairs_3d_noalloc, per GPU */
 
 /* Which reference engine a request replaces. */

@@ -243,9 +243,16 @@ def test_marked_xy_z_request_validation():
     req, keep = _request(E.HP_MARKED_NPAIRS_XY_Z, wfunc=16, nw=5)
     assert lib.hp_workspace_size(ctypes.byref(req)) == 0
     assert "1..15" in lib.hp_last_error().decode()
-    req, keep = _request(E.HP_MARKED_NPAIRS_XY_Z, nr=100, np_=100)
-    assert lib.hp_workspace_size(ctypes.byref(req)) == 0
-    assert "5800" in lib.hp_last_error().decode()
+    # any number of bins: wide histograms are counted over several launches
+    for nr, np_, ok in ((100, 100, True), (1 << 14, 1 << 13, False)):
+        req, keep = _request(E.HP_MARKED_NPAIRS_XY_Z, nr=nr, np_=np_)
+        r2 = np.linspace(1e-4, 0.01, nr)
+        p2 = np.linspace(0.0, 0.01, np_)
+        req.r2 = r2.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        req.p2 = p2.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        size = lib.hp_workspace_size(ctypes.byref(req))
+        assert (size > 0) == ok, lib.hp_last_error()
+    assert "2^26" in lib.hp_last_error().decode()
 
 
 def test_marked_weight_length_message():
@@ -354,8 +361,8 @@ def test_request_planning_sweep():
             mocks[m].pbc = int(rng.integers(0, 2))
             if rng.random() < 0.3:
                 mocks[m].cell[:] = [float(rng.uniform(-1, 60))] * 3
-        nr = int(rng.choice([1, 2, 3, 20, 64, 65, 128, 129, 511, 512, 513]))
-        np_ = int(rng.choice([0, 1, 2, 41, 100, 512, 513]))
+        nr = int(rng.choice([1, 2, 3, 20, 64, 65, 128, 129, 511, 512, 513, 3000]))
+        np_ = int(rng.choice([0, 1, 2, 41, 100, 512, 513, 5000]))
         r2 = np.sort(rng.uniform(0.0, 15.0, nr)) ** 2
         if rng.random() < 0.1:
             r2 = r2[::-1].copy()        # non-monotone edges are accepted (suffix minimum)
