@@ -1,5 +1,7 @@
 // pairs.hip -- workgroup shape and LDS layout of the pair kernels, and the
 // dispatch to the per-mode launchers.  The kernel itself is pairs_kernel.h.
+#include <cstdlib>
+
 #include "pairs_kernel.h"
 
 namespace hp {
@@ -98,11 +100,16 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
         else
             num_cus = 256;
     }
+    // product marks (weight_func_id 1, one weight per point) have a compiled-in variant
+    const int prod = (L.wfunc == 1 && L.nw == 1) ? 1 : 0;
+    // waves per SIMD the kernels' registers allow (their launch bounds; the
+    // per-lane-row Jacobian kernel is bounded by LDS long before registers)
     const int reg_waves =
         4 * (L.mode == kModeJac ? 6
-                                : ((L.mode == kModeWSum || L.mode == kModeTan || L.mode == kModeWXYZ ||
-                                    L.mode == kModeJack || L.mode == kModeJackXY) ? 4
-                                                                                                      : (L.mode == kModeXYZ ? 5 : HP_COUNT_WAVES)));
+                                : (((L.mode == kModeWSum || L.mode == kModeWXYZ) && prod) ? 5
+                                   : ((L.mode == kModeWSum || L.mode == kModeTan || L.mode == kModeWXYZ ||
+                                       L.mode == kModeJack || L.mode == kModeJackXY) ? 4
+                                                                                     : (L.mode == kModeXYZ ? 5 : HP_COUNT_WAVES))));
     // shape for a launch of kc bins: sets A's layout, returns the waves per
     // workgroup (0 if nothing fits) and the resident waves per CU
     auto shape = [&](int kc, int* resident) -> int {
@@ -160,7 +167,14 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
         A.kc = kn;
         A.smem = A.tab_bytes + waves * A.wave_bytes;
         A.L.pevals = k0 == 0 ? L.pevals : nullptr;  // every launch evaluates the same pairs
-        A.grid = num_cus * std::max(1, res / waves);   // workgroups resident at once
+        A.grid = num_cus;  // persistent launches: times the workgroups one CU holds (launch_t)
+        // below 2^32 by the largest batch of increments between two checks;
+        // HALOPAIR_SHARED_FLUSH_AT lowers it (tests exercise the flush path)
+        A.flush_at = 0xF0000000ull;
+        if (const char* e = std::getenv("HALOPAIR_SHARED_FLUSH_AT")) {
+            const unsigned long long v = std::strtoull(e, nullptr, 10);
+            if (v > 0 && v < A.flush_at) A.flush_at = v;
+        }
         if (!L.queue || launch_fill_u32(L.queue, 0u, 8, s)) return -1;
         int rc = -1;
         switch (L.mode) {
@@ -168,10 +182,10 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
             case kModeProj: rc = launch_mode<kModeProj>(A, waves, multi, also, 0, s); break;
             case kModeXYZ: rc = launch_mode<kModeXYZ>(A, waves, multi, also, 0, s); break;
             case kModeObj: rc = launch_mode<kModeObj>(A, waves, multi, also, 0, s); break;
-            case kModeWXYZ: rc = launch_mode<kModeWXYZ>(A, waves, multi, also, 0, s); break;
+            case kModeWXYZ: rc = launch_mode<kModeWXYZ>(A, waves, multi, also, prod, s); break;
             case kModeJack: rc = launch_mode<kModeJack>(A, waves, multi, also, 0, s); break;
             case kModeJackXY: rc = launch_mode<kModeJackXY>(A, waves, multi, also, 0, s); break;
-            case kModeWSum: rc = launch_mode<kModeWSum>(A, waves, multi, also, 0, s); break;
+            case kModeWSum: rc = launch_mode<kModeWSum>(A, waves, multi, also, prod, s); break;
             case kModeJac: rc = launch_mode<kModeJac>(A, waves, multi, also, L.swap, s); break;
             case kModeTan: rc = launch_mode<kModeTan>(A, waves, multi, also, 0, s); break;
         }

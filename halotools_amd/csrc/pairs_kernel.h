@@ -196,7 +196,8 @@ struct PairArgs {
     int jrow;           // jackknife: bytes of one subsample row of the wave's table
     int jr0_off;        // 3-D jackknife: byte offset of the all-pairs rows in 16 lane columns
     int k0, kc;         // this launch histograms the radial / rp start bins [k0, k0 + kc)
-    int grid;           // persistent launch: workgroups that fit on the device at once
+    int grid;           // persistent launch: compute units of the device
+    uint64_t flush_at;  // shared u32 histograms: flush once this many increments may have landed
 };
 
 // Survivor ring per wave: up to kBatch-1 leftover entries plus one 64-point chunk.
@@ -205,13 +206,16 @@ static_assert(kRing >= kBatch - 1 + 64, "survivor ring too small for the batch s
 
 // Occupancy the register allocation is held to (waves per SIMD): the count
 // kernels are limited by VGPRs otherwise, the FP64-histogram kernels by LDS.
-template <int MODE>
+template <int MODE, int VAR = 0>
 constexpr int kWavesPerSimd =
     (MODE == kModeJac) ? 1
-                       : ((MODE == kModeWSum || MODE == kModeTan || MODE == kModeWXYZ || MODE == kModeJack ||
-                           MODE == kModeJackXY)
-                              ? 4
-                              : (MODE == kModeXYZ ? 5 : HP_COUNT_WAVES));
+                       // product marks (compiled in) need fewer registers than the generic marking functions
+                       : (((MODE == kModeWSum || MODE == kModeWXYZ) && VAR) ? 5
+                          : ((MODE == kModeWSum || MODE == kModeTan || MODE == kModeWXYZ || MODE == kModeJack ||
+                              MODE == kModeJackXY)
+                                 ? 4
+                                 : (MODE == kModeXYZ ? 5 : HP_COUNT_WAVES)));
+
 // per-object counts: one u32 row set per lane (each lane owns its point's counts)
 
 // Modes whose workgroups are persistent (measured on MI355X: the FP64-histogram
@@ -225,8 +229,13 @@ constexpr bool kPersist = HP_PERSIST && (MODE == kModeWSum || MODE == kModeJac |
                                          MODE == kModeWXYZ || MODE == kModeJack || MODE == kModeJackXY ||
                                          HP_PERSIST_COUNT);
 
-template <int MODE, int SWAP, int ALSO, int MULTI, int WAVES>
-__global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const PairArgs A) {
+// VAR selects a variant of the mode: the Jacobian pass with the lanes on
+// sample 2 (kModeJac), or the product marking function w1[0] * w2[0] with one
+// weight per point compiled in (kModeWSum, kModeWXYZ).
+template <int MODE, int VAR, int ALSO, int MULTI, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pairs(const PairArgs A) {
+    constexpr int SWAP = (MODE == kModeJac) ? VAR : 0;
+    constexpr bool PROD = (MODE == kModeWSum || MODE == kModeWXYZ) && VAR;
     // jackknife: FP64 tables of (w1 w2) per bin, one row for all pairs and one
     // per subsample tag the wave meets (weights are (w, tag) rows)
     constexpr bool JK = (MODE == kModeJack || MODE == kModeJackXY);
@@ -400,8 +409,6 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
         const uint32_t trash_p = (uint32_t)Kp * cell_bytes;  // xy_z: beyond the last pi edge
 
         uint64_t evals = 0;
-        // Shared (per-wave) uint32 histograms can receive up to 64 increments per
-        // stream point; flush them into the group's uint64 row before they could wrap.
         const int shared_bins = (MODE == kModeXYZ) ? kc * Kp : kc;  // this launch's bins
         const int row_len = (MODE == kModeXYZ) ? Kp + 1 : 1;
         // per-mock count accumulator: integer adds are exact in any order, so the
@@ -586,8 +593,10 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                 if (MODE == kModeJac) {
                     v = sload_f64(SWm, r * 8u);
                 } else {
-                    double sw[5];
-                    for (int c = 0; c < nw && c < 5; c++) sw[c] = sload_f64(SWm, (r * (uint32_t)nw + (uint32_t)c) * 8u);
+                    double sw[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int c = 0; c < 5; c++)
+                        if (c < nw) sw[c] = sload_f64(SWm, (r * (uint32_t)nw + (uint32_t)c) * 8u);
                     v = wfunc_eval(wfunc, lw, sw);
                 }
                 __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a), v, __ATOMIC_RELAXED,
@@ -698,7 +707,10 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                 // Auto-count with one set of weights: the sample-2 rows need the
                 // swapped orientation of each pair, (x2 + x2shift) - x1, which
                 // across the periodic seam may round differently from the forward
-                // one; record the rare disagreements so rows2 = rows1 + corr exactly.
+                // one; record the rare disagreements so rows2 = rows1 + corr (the
+                // swapped pass's rows, exactly for sums of small integers and to
+                // the rounding of the row sums otherwise: corr is added with
+                // atomics from many waves, in no fixed order).
                 const uint32_t* __restrict__ SP = L.strm.perm;
 #pragma unroll
                 for (int u = 0; u < B; u++) {
@@ -766,6 +778,20 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                                                    1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
                 }
+            } else if (PROD) {
+                // product marks: the batch's stream weights in one scalar-load wait
+                double bw[B];
+#pragma unroll
+                for (int u = 0; u < B; u++) bw[u] = sload_f64(SWm, off[u]);
+#pragma unroll
+                for (int u = 0; u < B; u++) {
+                    if (XY && a[u] == trash_r) continue;
+                    __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a[u]), lw[0] * bw[u], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (ALSO)
+                        __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(wsm + A.cnt_off + a[u] + cnt_col), 1u,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
             } else {
 #pragma unroll
                 for (int u = 0; u < B; u++)
@@ -775,6 +801,25 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
         auto drain = [&]() __attribute__((always_inline)) {
             for (int q = 0; q < qlen; q += B) consume(q, min(B, qlen - q));
             qlen = 0;
+        };
+        // Shared (per-wave) uint32 histograms: each stream point adds at most
+        // 2 (an unshifted pair counted for both orders) for each of the 64
+        // lanes.  Before the points of another chunk could make a bin wrap,
+        // the histogram is flushed into the group's uint64 accumulator; the
+        // survivors still waiting in the ring land in the emptied histogram.
+        auto guard_shared = [&](uint64_t add) __attribute__((always_inline)) {
+            bound += add;
+            if (bound >= A.flush_at) {
+                for (int s = lane; s < shared_bins; s += 64) {
+                    const int src = (MODE == kModeXYZ) ? (s / Kp) * row_len + s % Kp : s;
+                    if (ch32[src])
+                        __hip_atomic_fetch_add(cacc + s, (uint64_t)ch32[src], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                }
+                for (int s = lane; s < (kc + 1) * row_len; s += 64) ch32[s] = 0u;
+                __builtin_amdgcn_wave_barrier();
+                bound = add + (uint64_t)qlen * 128u;
+            }
         };
 
         for (int jx = jx0; jx <= jx1; jx++) {
@@ -806,19 +851,6 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                     const int jb = uni((int)CS[colbase + w0]);
                     const int je = uni((int)CS[colbase + w1 + 1]);
                     if (jb >= je) continue;
-                    if (shared) {
-                        bound += (uint64_t)cnt * (uint64_t)(je - jb) * (sym ? 2u : 1u);
-                        if (bound >= 0xF0000000ull) {
-                            for (int s = lane; s < shared_bins; s += 64) {
-                                const int src = (MODE == kModeXYZ) ? (s / Kp) * row_len + s % Kp : s;
-                                if (ch32[src])
-                                    __hip_atomic_fetch_add(cacc + s, (uint64_t)ch32[src], __ATOMIC_RELAXED,
-                                                           __HIP_MEMORY_SCOPE_AGENT);
-                            }
-                            for (int s = lane; s < (kc + 1) * row_len; s += 64) ch32[s] = 0u;
-                            bound = (uint64_t)cnt * (uint64_t)(je - jb) * (sym ? 2u : 1u);
-                        }
-                    }
                     // x2shift of the reference: the shift that brings the sample-2
                     // point next to the sample-1 point.  Normal orientation visits
                     // sample-2 images (x2shift = t*L); swapped orientation visits
@@ -860,6 +892,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
                     const bool after_group = sym && img == 13;
                     const float ftx = (float)(tx * Lx), fty = (float)(ty * Ly), ftz = (float)(tz * Lz);
                     for (int c = j; c < je; c += 64) {
+                        if (shared) guard_shared((uint64_t)cnt * 128u);
                         const int jj = c + lane;
                         bool ok = false;
                         if (jj < je) {
@@ -901,6 +934,7 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
         drain();
 
         if (sym) {
+            if (shared) guard_shared((uint64_t)cnt * 128u);
             // the group's own points as stream points, unshifted: lane i with
             // point k counts 2 when k follows i in the lane order, 1 for k == i
             // (the self pair), 0 when k precedes i (counted from the other side)
@@ -1080,41 +1114,51 @@ __global__ void __launch_bounds__(WAVES * 64, kWavesPerSimd<MODE>) k_pairs(const
     }  // work loop
 }
 
-template <int MODE, int SWAP, int ALSO, int MULTI, int WAVES>
+template <int MODE, int VAR, int ALSO, int MULTI, int WAVES>
 static int launch_t(const PairArgs& A, hipStream_t s) {
     int64_t nblocks = (A.L.ngroups + WAVES - 1) / WAVES;
     if (nblocks <= 0) return 0;
-    if (kPersist<MODE>) nblocks = std::min<int64_t>(nblocks, A.grid);
-    hipLaunchKernelGGL((k_pairs<MODE, SWAP, ALSO, MULTI, WAVES>), dim3((unsigned)nblocks), dim3(WAVES * 64),
+    if (kPersist<MODE>) {
+        // as many workgroups as the device holds at once (registers, LDS, waves)
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pairs<MODE, VAR, ALSO, MULTI, WAVES>, WAVES * 64,
+                                                         (size_t)A.smem) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        nblocks = std::min<int64_t>(nblocks, (int64_t)A.grid * per_cu);
+    }
+    hipLaunchKernelGGL((k_pairs<MODE, VAR, ALSO, MULTI, WAVES>), dim3((unsigned)nblocks), dim3(WAVES * 64),
                        (size_t)A.smem, s, A);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int MODE, int SWAP, int ALSO, int MULTI>
+template <int MODE, int VAR, int ALSO, int MULTI>
 static int launch_w(const PairArgs& A, int waves, hipStream_t s) {
-    if (waves == 8) return launch_t<MODE, SWAP, ALSO, MULTI, 8>(A, s);
-    if (waves == 4) return launch_t<MODE, SWAP, ALSO, MULTI, 4>(A, s);
-    if (waves == 2) return launch_t<MODE, SWAP, ALSO, MULTI, 2>(A, s);
-    return launch_t<MODE, SWAP, ALSO, MULTI, 1>(A, s);
+    if (waves == 8) return launch_t<MODE, VAR, ALSO, MULTI, 8>(A, s);
+    if (waves == 4) return launch_t<MODE, VAR, ALSO, MULTI, 4>(A, s);
+    if (waves == 2) return launch_t<MODE, VAR, ALSO, MULTI, 2>(A, s);
+    return launch_t<MODE, VAR, ALSO, MULTI, 1>(A, s);
 }
 
-template <int MODE, int SWAP, int ALSO>
+template <int MODE, int VAR, int ALSO>
 static int launch_m(const PairArgs& A, int waves, int multi, hipStream_t s) {
-    if (multi) return launch_w<MODE, SWAP, ALSO, 1>(A, waves, s);
-    return launch_w<MODE, SWAP, ALSO, 0>(A, waves, s);
+    if (multi) return launch_w<MODE, VAR, ALSO, 1>(A, waves, s);
+    return launch_w<MODE, VAR, ALSO, 0>(A, waves, s);
 }
 
 // One launch of mode MODE with the workgroup shape and variant flags chosen
-// by launch_pairs (pairs.hip).  Each mode is instantiated in its own
+// by launch_pairs (pairs.hip); `var` is the kernel's VAR (swapped Jacobian
+// pass, or compiled-in product marks).  Each mode is instantiated in its own
 // translation unit (pairs_mode_*.hip) so the variants compile in parallel.
 template <int MODE>
-int launch_mode(const PairArgs& A, int waves, int multi, int also, int swap, hipStream_t s) {
-    if constexpr (MODE == kModeJac) {
-        if (swap)
+int launch_mode(const PairArgs& A, int waves, int multi, int also, int var, hipStream_t s) {
+    if constexpr (MODE == kModeJac || MODE == kModeWSum) {
+        if (var)
             return also ? launch_m<MODE, 1, 1>(A, waves, multi, s) : launch_m<MODE, 1, 0>(A, waves, multi, s);
         return also ? launch_m<MODE, 0, 1>(A, waves, multi, s) : launch_m<MODE, 0, 0>(A, waves, multi, s);
     }
-    if constexpr (MODE == kModeWSum || MODE == kModeTan)
+    if constexpr (MODE == kModeWXYZ)
+        return var ? launch_m<MODE, 1, 0>(A, waves, multi, s) : launch_m<MODE, 0, 0>(A, waves, multi, s);
+    if constexpr (MODE == kModeTan)
         return also ? launch_m<MODE, 0, 1>(A, waves, multi, s) : launch_m<MODE, 0, 0>(A, waves, multi, s);
     else
         return launch_m<MODE, 0, 0>(A, waves, multi, s);

@@ -98,7 +98,8 @@ typedef struct hp_request {
                                /* histograms do not fit in LDS)                        */
     const double* r2;          /* host [nr]: rbins*rbins (or rp_bins*rp_bins)          */
     const double* p2;          /* host [np]: pi_bins*pi_bins, or {pi_max*pi_max}        */
-    double search[3];          /* mesh search lengths (rmax,rmax,rmax) / (rp,rp,pi)     */
+    double search[3];          /* reserved, not read: the search lengths follow from   */
+                               /* the edges (sqrt of the largest r2 / p2)              */
     /* outputs, device */
     int64_t* counts;           /* [nmocks][nr] cumulative (xy_z: [nmocks][nr][np])      */
     double* sums;              /* marked: [nmocks][nr] cumulative weighted sums         */

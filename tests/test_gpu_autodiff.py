@@ -169,3 +169,30 @@ def test_seam_correction_matches_swapped_pass(gpu):
     # the two special points really are binned differently by the two orientations
     assert not np.array_equal(r1e[[10, 20]], r2e[[10, 20]])
     assert np.array_equal(elim.sums.cpu().numpy(), swapped.sums.cpu().numpy())
+
+
+def test_seam_correction_non_integer_weights(gpu):
+    # with non-integer weights rows2 = rows1 + corr equals the swapped pass to
+    # the rounding of the row sums (removing w_i from a rounded sum is not exact)
+    from halotools_amd import _device as D, _engine as E
+    xi, xj, r = _seam_pair()
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(0, 1, (800, 3))
+    pts[10] = [xi, 0.5, 0.5]
+    pts[20] = [xj, 0.5, 0.5]
+    w = rng.uniform(0.1, 2.0, 800)
+    rb = np.array([0.5 * r, r, 0.15, 0.2])
+    t = torch.tensor(pts.T.copy(), device=gpu)
+    cols = (t[0], t[1], t[2])
+    cols_copy = tuple(c.clone() for c in cols)
+    wt = torch.tensor(w, device=gpu).reshape(-1, 1)
+    mock = [dict(n1=800, n2=800, period=[1.0] * 3, pbc=True)]
+    elim = D.run(E.HP_MARKED_NPAIRS_3D, mock, cols, cols, rb * rb, same=True, w1=wt, nw=1, wfunc=1,
+                 want_counts=False, want_sums=True, rows1=True, rows2=True, device=gpu)
+    swapped = D.run(E.HP_MARKED_NPAIRS_3D, mock, cols, cols_copy, rb * rb, same=False, w1=wt, w2=wt.clone(), nw=1,
+                    wfunc=1, want_counts=False, want_sums=True, rows1=True, rows2=True, device=gpu)
+    torch.cuda.synchronize()
+    r2e, r2s = elim.rows2.cpu().numpy(), swapped.rows2.cpu().numpy()
+    np.testing.assert_array_equal(elim.rows1.cpu().numpy(), swapped.rows1.cpu().numpy())
+    np.testing.assert_allclose(r2e, r2s, rtol=1e-13, atol=1e-13 * np.abs(r2s).max())
+    np.testing.assert_allclose(elim.sums.cpu().numpy(), swapped.sums.cpu().numpy(), rtol=1e-13)

@@ -130,3 +130,15 @@ def test_kernel_rejects_bad_tags_without_faulting(pc, gpu):
                 n_samples=4, device=gpu)
     torch.cuda.synchronize()
     assert int(res.cbuf[-1].item()) & 2
+
+
+def test_weighted_tables_reproducible_to_rounding(pc):
+    # weighted subsample rows are added with FP64 atomics from many waves: two
+    # runs agree to rounding (relative to row 0), not necessarily bit for bit
+    s = random_sample(3000, seed=9)
+    t = _spatial_tags(s, 3)
+    w = np.random.default_rng(2).uniform(0.5, 1.5, 3000)
+    rb = np.array([0.02, 0.05, 0.1, 0.2])
+    a = pc.npairs_jackknife_3d(s, s, rb, t, t, 27, period=1.0, weights1=w, weights2=w)
+    b = pc.npairs_jackknife_3d(s, s, rb, t, t, 27, period=1.0, weights1=w, weights2=w)
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-13 * np.abs(a[0]).max())

@@ -153,3 +153,24 @@ def test_batched_step_many_bins(gpu):
         assert np.array_equal(tan[0][m].cpu().numpy(), ORC.npairs_3d(ph, ph, rb, period=L, num_threads=8))
     np.testing.assert_allclose(tan[1].cpu().numpy(), rev[1].cpu().numpy(), rtol=1e-12)
     np.testing.assert_allclose(tan[3].cpu().numpy(), rev[3].cpu().numpy(), rtol=1e-10)
+
+
+def test_shared_histogram_flush_path(mo):
+    """The per-wave u32 histograms (more than 64 radial bins, and every (rp, pi)
+    histogram) are flushed into 64-bit accumulators before a bin could wrap.
+    A lowered flush threshold (HALOPAIR_SHARED_FLUSH_AT, read at every launch)
+    makes every group flush many times; the counts must not change."""
+    import os
+    s1 = random_sample(4000, seed=31)
+    rb = _edges(100, 1e-3, 0.1)
+    rp, pi = _edges(12, 1e-3, 0.1), np.linspace(0.0, 0.2, 9)
+    ref3 = ORC.npairs_3d(s1, s1, rb, period=1.0, num_threads=8)
+    refx = ORC.npairs_xy_z(s1, s1, rp, pi, period=1.0, num_threads=8)
+    os.environ["HALOPAIR_SHARED_FLUSH_AT"] = "20000"
+    try:
+        got3 = mo.npairs_3d(s1, s1, rb, period=1.0)
+        gotx = mo.npairs_xy_z(s1, s1, rp, pi, period=1.0)
+    finally:
+        del os.environ["HALOPAIR_SHARED_FLUSH_AT"]
+    assert np.array_equal(got3, ref3)
+    assert np.array_equal(gotx, refx)
