@@ -175,7 +175,6 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
             const unsigned long long v = std::strtoull(e, nullptr, 10);
             if (v > 0 && v < A.flush_at) A.flush_at = v;
         }
-        if (!L.queue || launch_fill_u32(L.queue, 0u, 8, s)) return -1;
         int rc = -1;
         switch (L.mode) {
             case kModeCount3D: rc = launch_mode<kModeCount3D>(A, waves, multi, also, 0, s); break;

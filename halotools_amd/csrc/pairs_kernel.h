@@ -1125,6 +1125,7 @@ static int launch_t(const PairArgs& A, hipStream_t s) {
                                                          (size_t)A.smem) != hipSuccess || per_cu < 1)
             per_cu = 1;
         nblocks = std::min<int64_t>(nblocks, (int64_t)A.grid * per_cu);
+        if (!A.L.queue || launch_fill_u32(A.L.queue, 0u, 8, s)) return -1;  // fresh work queues
     }
     hipLaunchKernelGGL((k_pairs<MODE, VAR, ALSO, MULTI, WAVES>), dim3((unsigned)nblocks), dim3(WAVES * 64),
                        (size_t)A.smem, s, A);
