@@ -146,7 +146,9 @@ struct Plan {
     size_t o_pcount = 0, o_psum = 0, o_pevals = 0, o_red = 0, o_status = 0;
     bool elim = false;
     size_t o_corr = 0, o_rows_tmp = 0;
-    size_t o_hpos = 0, o_hrank = 0;  // symmetric auto-counts: Hilbert positions and ranks
+    size_t o_hrank = 0;              // symmetric auto-counts: Hilbert ranks of the row-major order
+    size_t o_rows[2] = {0, 0};       // packed coordinate rows of each side with a row-major order
+    int row_stride[2] = {4, 4}, row_nw[2] = {0, 0};
     size_t o_keys_row = 0;           // row-major keys of a side sorted in both orders
     bool sym = false;  // auto-count (unweighted or tangent): symmetric evaluation
     int crep = 1;
@@ -408,9 +410,15 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     // unweighted auto-counts evaluate each unshifted unordered pair once; they
     // need every row-major point's position in the Hilbert order
     P.sym = P.same && !P.obj && (!P.marked || P.tangent);
-    if (P.sym && P.n[0] > 0) {
-        P.o_hpos = take((size_t)P.n[0] * 4);
-        P.o_hrank = take((size_t)P.n[0] * 4);
+    if (P.sym && P.n[0] > 0) P.o_hrank = take((size_t)P.n[0] * 4);
+    // packed rows (x, y, z, Hilbert position[, up to 4 weights]) of every side
+    // the stream side is built from: one random row read per point in the gather
+    for (int s = 0; s < 2; s++) {
+        if (!P.sorted[s][kRowMajor].needed) continue;
+        const bool pack_w = P.weighted() && P.nw <= 4;
+        P.row_nw[s] = pack_w ? P.nw : 0;
+        P.row_stride[s] = pack_w ? 8 : 4;
+        P.o_rows[s] = take((size_t)std::max<int64_t>(P.n[s], 1) * (size_t)P.row_stride[s] * 8);
     }
     if (P.jack) {
         // (w, tag) rows of each side, and the per-mock subsample tables
@@ -518,14 +526,19 @@ static int run(const hp_request* r, int kind) {
             // a side sorted in both orders computes its keys once (Hilbert build first)
             const bool both = P.sorted[sd][kHilbert].needed && P.sorted[sd][kRowMajor].needed;
             uint32_t* krow = both ? reinterpret_cast<uint32_t*>(ws + P.o_keys_row) : nullptr;
-            uint32_t* hpos = rank ? reinterpret_cast<uint32_t*>(ws + P.o_hpos) : nullptr;
             uint32_t* hrank = rank ? reinterpret_cast<uint32_t*>(ws + P.o_hrank) : nullptr;
+            PackedRows rows;
+            if (P.sorted[sd][kRowMajor].needed) {
+                rows.p = reinterpret_cast<double*>(ws + P.o_rows[sd]);
+                rows.stride = P.row_stride[sd];
+                rows.nw = P.row_nw[sd];
+            }
             static_assert(kHilbert < kRowMajor, "the Hilbert order must be built before the row-major order");
             if (build_sorted_side(dgeom, P.M, sd, (Order)o, X[sd], Y[sd], Z[sd], P.weighted() ? W[sd] : nullptr, P.nw,
                                   P.n[sd], P.total_cells, o == kRowMajor ? P.bits_row : P.bits_hilbert, sb,
-                                  side_ptrs(ws, S), status, s, o == kHilbert ? hpos : nullptr,
-                                  o == kRowMajor ? hpos : nullptr, o == kRowMajor ? hrank : nullptr,
-                                  o == kHilbert ? krow : nullptr, o == kRowMajor ? krow : nullptr))
+                                  side_ptrs(ws, S), status, s, rows, rank && o == kHilbert,
+                                  o == kRowMajor ? hrank : nullptr, o == kHilbert ? krow : nullptr,
+                                  o == kRowMajor ? krow : nullptr))
                 return fail("cell build launch failed");
         }
     }

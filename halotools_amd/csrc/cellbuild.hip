@@ -95,19 +95,33 @@ __device__ __forceinline__ int cell_index(double v, double cell, int n, bool& ba
 }
 
 // keys_row: when the same side is also sorted in row-major order, the
-// Hilbert pass writes those keys too (one read of the coordinates for both)
+// Hilbert pass writes those keys too (one read of the coordinates for both).
+// rows: the point's packed row (coordinates, and weights when they fit).
 __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, int order,
                        const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
                        int64_t n, uint32_t* __restrict__ keys, int32_t* __restrict__ status,
-                       uint32_t* __restrict__ keys_row) {
+                       uint32_t* __restrict__ keys_row, const double* __restrict__ w, int nw, PackedRows rows) {
     const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int64_t m = mock_of_point(geom, nmocks, side, p);
     const MockGeom& G = geom[m];
+    const double px = x[p], py = y[p], pz = z[p];
+    if (rows.p) {
+        double* r = rows.p + p * rows.stride;
+        reinterpret_cast<double4*>(r)[0] = make_double4(px, py, pz, 0.0);
+        if (rows.nw > 0) {
+            double4 wr = make_double4(0.0, 0.0, 0.0, 0.0);
+            wr.x = w[p * nw];
+            if (rows.nw > 1) wr.y = w[p * nw + 1];
+            if (rows.nw > 2) wr.z = w[p * nw + 2];
+            if (rows.nw > 3) wr.w = w[p * nw + 3];
+            reinterpret_cast<double4*>(r)[1] = wr;
+        }
+    }
     bool bad = false;
-    const int ix = cell_index(x[p], G.cell[0], G.ndiv[0], bad, G.period[0], G.pbc);
-    const int iy = cell_index(y[p], G.cell[1], G.ndiv[1], bad, G.period[1], G.pbc);
-    const int iz = cell_index(z[p], G.cell[2], G.ndiv[2], bad, G.period[2], G.pbc);
+    const int ix = cell_index(px, G.cell[0], G.ndiv[0], bad, G.period[0], G.pbc);
+    const int iy = cell_index(py, G.cell[1], G.ndiv[1], bad, G.period[1], G.pbc);
+    const int iz = cell_index(pz, G.cell[2], G.ndiv[2], bad, G.period[2], G.pbc);
     uint32_t key;
     const uint32_t rkey = (uint32_t)(G.cell_off + ((int64_t)ix * G.ndiv[1] + iy) * G.ndiv[2] + iz);
     if (order == kRowMajor) {
@@ -119,9 +133,9 @@ __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, 
         const int hb = G.mbits;
         const int hn = 1 << hb;
         bool dummy = false;
-        const int hx = cell_index(x[p], G.period[0] / hn, hn, dummy, G.period[0], 0);
-        const int hy = cell_index(y[p], G.period[1] / hn, hn, dummy, G.period[1], 0);
-        const int hz = cell_index(z[p], G.period[2] / hn, hn, dummy, G.period[2], 0);
+        const int hx = cell_index(px, G.period[0] / hn, hn, dummy, G.period[0], 0);
+        const int hy = cell_index(py, G.period[1] / hn, hn, dummy, G.period[1], 0);
+        const int hz = cell_index(pz, G.period[2] / hn, hn, dummy, G.period[2], 0);
         key = (uint32_t)(G.key_off + hilbert3((uint32_t)hx, (uint32_t)hy, (uint32_t)hz, hb));
     }
     keys[p] = key;
@@ -183,7 +197,7 @@ __global__ void __launch_bounds__(256) k_radix_scatter(const uint32_t* __restric
                                                        uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                        int64_t n, int shift, const uint32_t* __restrict__ tile_counts,
                                                        const uint32_t* __restrict__ digit_totals, int ntiles,
-                                                       uint32_t* __restrict__ inv) {
+                                                       double* __restrict__ inv_rows, int inv_stride) {
     __shared__ uint32_t skey[kSortTile];
     __shared__ uint32_t sval[kSortTile];
     __shared__ uint32_t wcnt[kScatterWaves][kRadixDigits];  // per-wave digit counts, then offsets
@@ -287,24 +301,36 @@ __global__ void __launch_bounds__(256) k_radix_scatter(const uint32_t* __restric
         const uint32_t pos = gbase[d] + (uint32_t)i - toff[d];
         kout[pos] = key;
         vout[pos] = sval[i];
-        if (inv) inv[sval[i]] = pos;  // last pass: the sorted position of each input point
+        // last pass: the sorted position of each input point, into its packed row
+        if (inv_rows) reinterpret_cast<uint32_t*>(inv_rows + (int64_t)sval[i] * inv_stride)[6] = pos;
     }
 }
 
-// Sorted copies of coordinates (and weight rows) in the new order.
-__global__ void k_gather(const uint32_t* __restrict__ perm, int64_t n, const double* __restrict__ x,
-                         const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ w,
+// Sorted copies of coordinates (and weight rows) in the new order: one packed
+// row per point (x, y, z, the Hilbert position, and the weights when they
+// fit in the row) read at its input position.
+__global__ void k_gather(const uint32_t* __restrict__ perm, int64_t n, PackedRows rows, const double* __restrict__ w,
                          int nw, double* __restrict__ ox, double* __restrict__ oy, double* __restrict__ oz,
-                         double* __restrict__ ow, const uint32_t* __restrict__ hpos, uint32_t* __restrict__ hrank) {
+                         double* __restrict__ ow, uint32_t* __restrict__ hrank) {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q >= n) return;
     const int64_t p = perm[q];
-    if (hrank) hrank[q] = hpos[p];
-    ox[q] = x[p];
-    oy[q] = y[p];
-    oz[q] = z[p];
-    if (w) {
-        for (int c = 0; c < nw; c++) ow[q * nw + c] = w[p * nw + c];
+    const double* r = rows.p + p * rows.stride;
+    const double4 c = reinterpret_cast<const double4*>(r)[0];
+    if (hrank) hrank[q] = (uint32_t)__double2loint(c.w);
+    ox[q] = c.x;
+    oy[q] = c.y;
+    oz[q] = c.z;
+    if (ow) {
+        if (rows.nw > 0) {
+            const double4 wr = reinterpret_cast<const double4*>(r)[1];
+            ow[q * nw] = wr.x;
+            if (nw > 1) ow[q * nw + 1] = wr.y;
+            if (nw > 2) ow[q * nw + 2] = wr.z;
+            if (nw > 3) ow[q * nw + 3] = wr.w;
+        } else {
+            for (int k = 0; k < nw; k++) ow[q * nw + k] = w[p * nw + k];
+        }
     }
 }
 
@@ -322,9 +348,8 @@ static inline int check(const char*) { return hipGetLastError() == hipSuccess ? 
 
 int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, const double* x, const double* y,
                       const double* z, const double* w, int nw, int64_t n, int64_t total_cells, int key_bits,
-                      SortBufs sb, SortedSide out, int32_t* status, hipStream_t s, uint32_t* hpos_out,
-                      const uint32_t* hpos_in, uint32_t* hrank_out, uint32_t* keys_row_out,
-                      const uint32_t* keys_in) {
+                      SortBufs sb, SortedSide out, int32_t* status, hipStream_t s, PackedRows rows,
+                      bool hpos_to_rows, uint32_t* hrank_out, uint32_t* keys_row_out, const uint32_t* keys_in) {
     if (n <= 0) {
         if (order == kRowMajor) return launch_fill_u32(out.cell_start, 0u, total_cells + 1, s);
         return 0;
@@ -333,7 +358,7 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, c
     // keys_in: this order's keys were written by the other order's key pass
     if (!keys_in)
         hipLaunchKernelGGL(k_keys, dim3(nb), dim3(256), 0, s, geom, nmocks, side, (int)order, x, y, z, n, sb.keys[0],
-                           status, order == kHilbert ? keys_row_out : (uint32_t*)nullptr);
+                           status, order == kHilbert ? keys_row_out : (uint32_t*)nullptr, w, nw, rows);
     const int ntiles = (int)((n + kSortTile - 1) / kSortTile);
     const int passes = key_bits <= 0 ? 1 : (key_bits + kRadixBits - 1) / kRadixBits;
     int cur = 0;
@@ -349,15 +374,17 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, c
         hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(256), 0, s, kin,
                            pass == 0 ? (const uint32_t*)nullptr : sb.vals[cur], sb.keys[cur ^ 1], vdst, n,
                            shift, sb.tile_counts, sb.digit_totals, ntiles,
-                           pass == passes - 1 ? hpos_out : (uint32_t*)nullptr);
+                           (pass == passes - 1 && hpos_to_rows) ? rows.p : (double*)nullptr, rows.stride);
         cur ^= 1;
     }
     if (check("sort")) return -1;
     // The lane side (Hilbert order) is read once per group through the
     // permutation; only the stream side needs sorted copies.
-    if (order == kRowMajor)
-        hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, s, out.perm, n, x, y, z, w, nw, out.x, out.y, out.z,
-                           out.w, hpos_in, hrank_out);
+    if (order == kRowMajor) {
+        if (!rows.p) return -1;  // the row-major gather reads the packed rows
+        hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, s, out.perm, n, rows, w, nw, out.x, out.y, out.z,
+                           out.w, hrank_out);
+    }
     if (order == kRowMajor) {
         hipLaunchKernelGGL(k_cell_start, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, sb.keys[cur], n,
                            total_cells, out.cell_start);

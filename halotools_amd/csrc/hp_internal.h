@@ -80,25 +80,39 @@ struct SortedSide {
     uint32_t* cell_start;  // row-major order only: [total_cells + 1]
 };
 
+// One side's coordinates (and up to four weights) packed as fixed-size rows
+// in input order: written by the side's first key pass (which reads the
+// coordinates anyway) and read by the row-major gather, so that gather reads
+// one row per point at a random position instead of one element from each
+// of the x, y, z (and weight) arrays.  The fourth double of a row holds, in
+// its low word, the point's position in the Hilbert order (written by the
+// Hilbert build's last radix pass when a symmetric auto-count needs it).
+struct PackedRows {
+    double* p = nullptr;  // [n][stride], or nullptr: no packed rows
+    int stride = 4;       // doubles per row: 4 (x, y, z, slot) or 8 (x, y, z, slot, w0..w3)
+    int nw = 0;           // weights packed from the fifth double on (0: gathered from w)
+};
+
 // ------------------------------------------------------------------ launchers
 // cellbuild.hip
 int launch_store_bytes(void* dst, const void* src, size_t nbytes, hipStream_t s);
 int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 int launch_group_mock(const MockGeom* g, int nmocks, int side, int32_t* group_mock,
                       int64_t ngroups_total, hipStream_t s);
-// Symmetric auto-counts also need, for every point of the row-major order,
-// its position in the Hilbert order: the Hilbert build's last radix pass
-// writes that position per point (hpos_out), the row-major gather reads it
-// through its permutation (hpos_in -> hrank_out).  When one side is sorted in
-// both orders, the Hilbert build also writes the row-major keys
-// (keys_row_out) and the row-major build starts from them (keys_in).  Null
-// pointers skip each of these.
+// The side's key pass (the first build of the side) writes its packed rows
+// (`rows`, when rows.p is set and keys_in is null).  Symmetric auto-counts
+// also need, for every point of the row-major order, its position in the
+// Hilbert order: the Hilbert build's last radix pass writes that position
+// into the point's row (hpos_to_rows), the row-major gather reads it through
+// its permutation (hrank_out).  When one side is sorted in both orders, the
+// Hilbert build also writes the row-major keys (keys_row_out) and the
+// row-major build starts from them (keys_in).  Null pointers skip each of
+// these.
 int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order,
                       const double* x, const double* y, const double* z, const double* w,
                       int nw, int64_t n, int64_t total_cells, int key_bits,
-                      SortBufs sb, SortedSide out, int32_t* status, hipStream_t s,
-                      uint32_t* hpos_out = nullptr, const uint32_t* hpos_in = nullptr,
-                      uint32_t* hrank_out = nullptr, uint32_t* keys_row_out = nullptr,
+                      SortBufs sb, SortedSide out, int32_t* status, hipStream_t s, PackedRows rows,
+                      bool hpos_to_rows = false, uint32_t* hrank_out = nullptr, uint32_t* keys_row_out = nullptr,
                       const uint32_t* keys_in = nullptr);
 
 // pairs.hip
