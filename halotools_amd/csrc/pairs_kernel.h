@@ -141,14 +141,6 @@ __device__ __forceinline__ uint32_t bin_addr(double d, const LdsEntry* __restric
     return chunk_addr(k, k0, kc, step);
 }
 
-__device__ __forceinline__ int64_t xcd_remap(int64_t bid, int64_t nb) {
-    // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-    // contiguous run of (Hilbert-ordered) groups so its L2 sees one region.
-    const int64_t q = nb / 8, r = nb % 8;
-    const int64_t x = bid % 8, k = bid / 8;
-    return x * q + (x < r ? x : r) + k;
-}
-
 #ifndef HP_BATCH
 #define HP_BATCH 8
 #endif
@@ -221,6 +213,9 @@ constexpr int kWavesPerSimd =
 // Modes whose workgroups are persistent (measured on MI355X: the FP64-histogram
 // passes gain 2-7 %; the integer count passes lose 8-26 %, their register
 // budget leaving no room for the work loop's state without spilling).
+#ifndef HP_QUEUES
+#define HP_QUEUES 8  // work queues of a persistent launch (one per XCD)
+#endif
 #ifndef HP_PERSIST_COUNT
 #define HP_PERSIST_COUNT 0
 #endif
@@ -312,15 +307,18 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
     // XCDs round-robin, so blockIdx % 8 is the XCD): queue x hands out the
     // x-th contiguous eighth of the (Hilbert-ordered) groups, so an XCD's L2
     // sees one region; a wave whose queue is empty moves on to the next
-    // XCD's.  Every wave leaves once all eight are exhausted.
+    // XCD's.  Every wave leaves once all eight are exhausted (one shared
+    // queue measured 4-13 % slower).  Other modes launch one wave per lane
+    // group, in group order (a remap giving each XCD a contiguous run of
+    // groups measured equal on one box and 8 % slower on the 64-mock batch).
     int qn = 0;
     const int xq = (int)(blockIdx.x & 7u);
     for (;;) {
         int64_t g = -1;
         if (kPersist<MODE>) {
-            while (qn < 8) {
-                const int q = (xq + qn) & 7;
-                const int64_t lo = L.ngroups * q / 8, hi = L.ngroups * (q + 1) / 8;
+            while (qn < HP_QUEUES) {
+                const int q = (xq + qn) % HP_QUEUES;
+                const int64_t lo = L.ngroups * q / HP_QUEUES, hi = L.ngroups * (q + 1) / HP_QUEUES;
                 uint32_t i = 0;
                 if (lane == 0) i = __hip_atomic_fetch_add(L.queue + q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 i = (uint32_t)uni((int)i);
@@ -336,7 +334,7 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
             for (int i = lane; i < A.wave_bytes / 4; i += 64) h[i] = 0u;
             __builtin_amdgcn_wave_barrier();
         } else {
-            g = xcd_remap(blockIdx.x, gridDim.x) * WAVES + wid;
+            g = (int64_t)blockIdx.x * WAVES + wid;
             if (g >= L.ngroups) break;
         }
         const double* gedges_r = reinterpret_cast<const double*>(static_cast<const char*>(L.ws) + L.br.edges_off);
