@@ -289,6 +289,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
             if (nd > 1024.0) nd = 1024.0;
             G.ndiv[d] = (int32_t)nd;
             G.cell[d] = L / (double)G.ndiv[d];
+            G.inv_cell[d] = 1.0 / G.cell[d];
             G.period[d] = L;
             Lmax = std::max(Lmax, L);
             ncells *= G.ndiv[d];
@@ -300,6 +301,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
         int mb = 1;
         while (mb < kHilbertMaxBits && (double)(1 << mb) < 1.25 * cbrt(nlane)) mb++;
         G.mbits = mb;
+        for (int d = 0; d < 3; d++) G.inv_hcell[d] = (double)(1 << mb) / G.period[d];
         const float eps = (float)(1e-5 * Lmax);
         G.eps = eps;
         G.search[0] = (float)(Rxy * (1.0 + 1e-6)) + eps;

@@ -3,7 +3,7 @@
 // points of each sample into a uniform grid).
 //
 // The build is a stable LSD counting sort of 32-bit cell keys (8-bit digits,
-// one wavefront per 2048-key tile), so the order of points inside a cell is
+// tiles of 2048 or 8192 keys), so the order of points inside a cell is
 // their input order and every downstream floating-point sum is run-to-run
 // deterministic.  Two key flavours exist:
 //   row-major  (ix*ny + iy)*nz + iz  -- stream side; the cells of one (ix, iy)
@@ -12,6 +12,7 @@
 //              spatially compact group (the curve has no jumps).
 // A key of mock m is offset by that mock's first cell (row-major) or first
 // Hilbert index, so one sort handles a whole batch of realisations.
+#include <cstdlib>
 #include <cstring>
 
 #include "hp_internal.h"
@@ -83,12 +84,14 @@ int launch_group_mock(const MockGeom* g, int nmocks, int side, int32_t* gm, int6
 // ----------------------------------------------------------------- cell keys
 // Cell index of a coordinate: floor(v / cell), clamped into [0, n-1] so that
 // v == period lands in the last cell (as PC/rectangular_mesh.py:19-22 does).
-// The pair kernels plan their windows with a safety margin, so the exact
-// rounding of this division does not affect which pairs are counted.
-__device__ __forceinline__ int cell_index(double v, double cell, int n, bool& bad, double period, int pbc) {
+// The quotient is formed as v * (1 / cell): the pair kernels plan their
+// windows with a safety margin, so the rounding of the quotient (a point
+// within rounding of a cell face may land in the cell next to it) does not
+// affect which pairs are counted.
+__device__ __forceinline__ int cell_index(double v, double inv_cell, int n, bool& bad, double period, int pbc) {
     if (!(v == v) || v > 1.0e300 || v < -1.0e300) { bad = true; return 0; }
     if (pbc && (v < 0.0 || v > period)) bad = true;
-    const double f = floor(v / cell);
+    const double f = floor(v * inv_cell);
     if (!(f >= 0.0)) return 0;
     if (f >= (double)n) return n - 1;
     return (int)f;
@@ -103,7 +106,13 @@ __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, 
                        uint32_t* __restrict__ keys_row, const double* __restrict__ w, int nw, PackedRows rows) {
     const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const int64_t m = mock_of_point(geom, nmocks, side, p);
+    // the mock of the point: a wave's 64 consecutive points nearly always lie
+    // in one mock, found by two wave-uniform searches (scalar loads); a wave
+    // that straddles mocks searches per lane
+    const int pf = __builtin_amdgcn_readfirstlane((int)p);
+    const int64_t mf = mock_of_point(geom, nmocks, side, pf);
+    const int64_t ml = mock_of_point(geom, nmocks, side, min((int64_t)pf + 63, n - 1));
+    const int64_t m = (mf == ml) ? mf : mock_of_point(geom, nmocks, side, p);
     const MockGeom& G = geom[m];
     const double px = x[p], py = y[p], pz = z[p];
     if (rows.p) {
@@ -119,9 +128,9 @@ __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, 
         }
     }
     bool bad = false;
-    const int ix = cell_index(px, G.cell[0], G.ndiv[0], bad, G.period[0], G.pbc);
-    const int iy = cell_index(py, G.cell[1], G.ndiv[1], bad, G.period[1], G.pbc);
-    const int iz = cell_index(pz, G.cell[2], G.ndiv[2], bad, G.period[2], G.pbc);
+    const int ix = cell_index(px, G.inv_cell[0], G.ndiv[0], bad, G.period[0], G.pbc);
+    const int iy = cell_index(py, G.inv_cell[1], G.ndiv[1], bad, G.period[1], G.pbc);
+    const int iz = cell_index(pz, G.inv_cell[2], G.ndiv[2], bad, G.period[2], G.pbc);
     uint32_t key;
     const uint32_t rkey = (uint32_t)(G.cell_off + ((int64_t)ix * G.ndiv[1] + iy) * G.ndiv[2] + iz);
     if (order == kRowMajor) {
@@ -133,9 +142,9 @@ __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, 
         const int hb = G.mbits;
         const int hn = 1 << hb;
         bool dummy = false;
-        const int hx = cell_index(px, G.period[0] / hn, hn, dummy, G.period[0], 0);
-        const int hy = cell_index(py, G.period[1] / hn, hn, dummy, G.period[1], 0);
-        const int hz = cell_index(pz, G.period[2] / hn, hn, dummy, G.period[2], 0);
+        const int hx = cell_index(px, G.inv_hcell[0], hn, dummy, G.period[0], 0);
+        const int hy = cell_index(py, G.inv_hcell[1], hn, dummy, G.period[1], 0);
+        const int hz = cell_index(pz, G.inv_hcell[2], hn, dummy, G.period[2], 0);
         key = (uint32_t)(G.key_off + hilbert3((uint32_t)hx, (uint32_t)hy, (uint32_t)hz, hb));
     }
     keys[p] = key;
@@ -143,15 +152,34 @@ __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, 
 }
 
 // ----------------------------------------------------- LSD counting sort
+// Tiles of TILE keys: per-tile digit counts, an exclusive scan of each digit's
+// counts over the tiles, then a scatter of every tile in digit order.  Large
+// sorts take bigger tiles: the per-tile digit counts are read and written at
+// 256 scattered words per tile, and a tile's keys leave in 256 runs, so 8192
+// keys per tile (runs of ~32 keys, full cache lines) move a fraction of the
+// bytes of 2048-key tiles (runs of ~8 keys) at tens of millions of points.
+template <int TILE>
 __global__ void __launch_bounds__(256) k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
                                                     uint32_t* __restrict__ tile_counts, int ntiles) {
     __shared__ uint32_t h[kRadixDigits];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kSortTile;
-    for (int i = threadIdx.x; i < kSortTile; i += 256) {
-        const int64_t idx = base + i;
-        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (kRadixDigits - 1)], 1u);
+    const int64_t base = (int64_t)blockIdx.x * TILE;
+    if (base + TILE <= n) {
+        // full tile: 16-byte loads
+        const uint4* k4 = reinterpret_cast<const uint4*>(keys + base);
+        for (int i = threadIdx.x; i < TILE / 4; i += 256) {
+            const uint4 k = k4[i];
+            atomicAdd(&h[(k.x >> shift) & (kRadixDigits - 1)], 1u);
+            atomicAdd(&h[(k.y >> shift) & (kRadixDigits - 1)], 1u);
+            atomicAdd(&h[(k.z >> shift) & (kRadixDigits - 1)], 1u);
+            atomicAdd(&h[(k.w >> shift) & (kRadixDigits - 1)], 1u);
+        }
+    } else {
+        for (int i = threadIdx.x; i < TILE; i += 256) {
+            const int64_t idx = base + i;
+            if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (kRadixDigits - 1)], 1u);
+        }
     }
     __syncthreads();
     tile_counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
@@ -185,40 +213,60 @@ __global__ void __launch_bounds__(256) k_radix_scan(uint32_t* __restrict__ tile_
     if (threadIdx.x == 255) digit_totals[blockIdx.x] = part[255];
 }
 
-// One workgroup (4 wavefronts) per tile of kSortTile keys.  Each wave ranks
-// its 512 keys stably (a lane's rank among lanes with the same digit comes
-// from eight ballots; per-wave digit counters carry it across rounds), the
-// tile is staged in LDS in digit order, and the staged keys go out in runs
-// of equal digits, so consecutive threads write consecutive addresses.
-constexpr int kScatterWaves = 4;
-constexpr int kPerWave = kSortTile / kScatterWaves;  // 512 keys, 8 rounds of 64
+// Inclusive Hillis-Steele scan of v over the first 256 threads of the block
+// (every thread of the block calls it; threads >= 256 pass 0 and get 0).
+template <int NT>
+__device__ __forceinline__ uint32_t scan_digits(uint32_t* scan, uint32_t v) {
+    const int d = threadIdx.x;
+    if (d < kRadixDigits) scan[d] = v;
+    __syncthreads();
+    for (int off = 1; off < kRadixDigits; off <<= 1) {
+        const uint32_t u = (d < kRadixDigits && d >= off) ? scan[d - off] : 0u;
+        __syncthreads();
+        if (d < kRadixDigits) scan[d] += u;
+        __syncthreads();
+    }
+    return d < kRadixDigits ? scan[d] : 0u;
+}
 
-__global__ void __launch_bounds__(256) k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                       uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                       int64_t n, int shift, const uint32_t* __restrict__ tile_counts,
-                                                       const uint32_t* __restrict__ digit_totals, int ntiles,
-                                                       double* __restrict__ inv_rows, int inv_stride) {
-    __shared__ uint32_t skey[kSortTile];
-    __shared__ uint32_t sval[kSortTile];
-    __shared__ uint32_t wcnt[kScatterWaves][kRadixDigits];  // per-wave digit counts, then offsets
-    __shared__ uint32_t toff[kRadixDigits];                 // first staged position of each digit
-    __shared__ uint32_t gbase[kRadixDigits];                // global position of the tile's first key per digit
+// One workgroup of WAVES wavefronts per tile of TILE keys.  Each wave ranks
+// its TILE / WAVES keys stably (a lane's rank among lanes with the same digit
+// comes from eight ballots; per-wave digit counters carry it across rounds),
+// the tile is staged in LDS in digit order, and the staged keys go out in
+// runs of equal digits, so consecutive threads write consecutive addresses.
+template <int TILE, int WAVES>
+__global__ void __launch_bounds__(WAVES * 64) k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                              const uint32_t* __restrict__ vin,
+                                                              uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                              int64_t n, int shift,
+                                                              const uint32_t* __restrict__ tile_counts,
+                                                              const uint32_t* __restrict__ digit_totals, int ntiles,
+                                                              double* __restrict__ inv_rows, int inv_stride) {
+    constexpr int NT = WAVES * 64;
+    constexpr int kPerWave = TILE / WAVES;
+    constexpr int kRounds = kPerWave / 64;
+    static_assert(NT >= kRadixDigits && kPerWave % 64 == 0, "tile shape");
+    __shared__ uint32_t skey[TILE];
+    __shared__ uint32_t sval[TILE];
+    __shared__ uint32_t wcnt[WAVES][kRadixDigits];  // per-wave digit counts, then offsets
+    __shared__ uint32_t toff[kRadixDigits];         // first staged position of each digit
+    __shared__ uint32_t gbase[kRadixDigits];        // global position of the tile's first key per digit
     __shared__ uint32_t scan[kRadixDigits];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wv = tid >> 6;
     const int tile = blockIdx.x;
-    const int64_t base = (int64_t)tile * kSortTile;
-    const int tn = (int)min((int64_t)kSortTile, n - base);
+    const int64_t base = (int64_t)tile * TILE;
+    const int tn = (int)min((int64_t)TILE, n - base);
 
-    for (int i = tid; i < kScatterWaves * kRadixDigits; i += 256) (&wcnt[0][0])[i] = 0u;
+    for (int i = tid; i < WAVES * kRadixDigits; i += NT) (&wcnt[0][0])[i] = 0u;
     __syncthreads();
 
     // ---- stable ranks within each wave
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    uint32_t kk[kPerWave / 64], vv[kPerWave / 64], rk[kPerWave / 64];
+    uint32_t kk[kRounds], vv[kRounds], rk[kRounds];
 #pragma unroll
-    for (int r = 0; r < kPerWave / 64; r++) {
+    for (int r = 0; r < kRounds; r++) {
         const int li = wv * kPerWave + r * 64 + lane;  // position in the tile
         const bool valid = li < tn;
         const uint32_t key = valid ? kin[base + li] : 0u;
@@ -243,62 +291,45 @@ __global__ void __launch_bounds__(256) k_radix_scatter(const uint32_t* __restric
     __syncthreads();
 
     // ---- digit offsets in the staged tile: digits in order, waves in order
-    {
-        const int d = tid;  // 256 threads, one digit each
-        uint32_t tot = 0;
-        for (int w = 0; w < kScatterWaves; w++) tot += wcnt[w][d];
-        scan[d] = tot;
-        __syncthreads();
-        for (int off = 1; off < kRadixDigits; off <<= 1) {
-            const uint32_t v = d >= off ? scan[d - off] : 0u;
-            __syncthreads();
-            scan[d] += v;
-            __syncthreads();
-        }
-        const uint32_t start = scan[d] - tot;
+    const int d = tid;  // the first 256 threads own one digit each
+    uint32_t tot = 0;
+    if (d < kRadixDigits)
+        for (int w = 0; w < WAVES; w++) tot += wcnt[w][d];
+    const uint32_t incl = scan_digits<NT>(scan, tot);
+    if (d < kRadixDigits) {
+        const uint32_t start = incl - tot;
         toff[d] = start;
         uint32_t run = start;
-        for (int w = 0; w < kScatterWaves; w++) {
+        for (int w = 0; w < WAVES; w++) {
             const uint32_t c = wcnt[w][d];
             wcnt[w][d] = run;
             run += c;
         }
-        // global base: all keys of smaller digits, then this digit's keys of earlier tiles
-        gbase[d] = tile_counts[(int64_t)d * ntiles + tile];
     }
-    {
-        // exclusive scan of the 256 digit totals (same for every tile)
-        const int d = tid;
-        const uint32_t t = digit_totals[d];
-        __syncthreads();
-        scan[d] = t;
-        __syncthreads();
-        for (int off = 1; off < kRadixDigits; off <<= 1) {
-            const uint32_t v = d >= off ? scan[d - off] : 0u;
-            __syncthreads();
-            scan[d] += v;
-            __syncthreads();
-        }
-        gbase[d] += scan[d] - t;
-    }
+    // global base: all keys of smaller digits (an exclusive scan of the digit
+    // totals, the same for every tile), then this digit's keys of earlier tiles
+    const uint32_t t = d < kRadixDigits ? digit_totals[d] : 0u;
+    __syncthreads();
+    const uint32_t tincl = scan_digits<NT>(scan, t);
+    if (d < kRadixDigits) gbase[d] = tile_counts[(int64_t)d * ntiles + tile] + tincl - t;
     __syncthreads();
 
     // ---- stage in digit order
 #pragma unroll
-    for (int r = 0; r < kPerWave / 64; r++) {
+    for (int r = 0; r < kRounds; r++) {
         if (rk[r] == 0xffffffffu) continue;
-        const uint32_t d = (kk[r] >> shift) & (kRadixDigits - 1);
-        const uint32_t pos = wcnt[wv][d] + rk[r];
+        const uint32_t dg = (kk[r] >> shift) & (kRadixDigits - 1);
+        const uint32_t pos = wcnt[wv][dg] + rk[r];
         skey[pos] = kk[r];
         sval[pos] = vv[r];
     }
     __syncthreads();
 
     // ---- write out: runs of equal digits land on consecutive addresses
-    for (int i = tid; i < tn; i += 256) {
+    for (int i = tid; i < tn; i += NT) {
         const uint32_t key = skey[i];
-        const uint32_t d = (key >> shift) & (kRadixDigits - 1);
-        const uint32_t pos = gbase[d] + (uint32_t)i - toff[d];
+        const uint32_t dg = (key >> shift) & (kRadixDigits - 1);
+        const uint32_t pos = gbase[dg] + (uint32_t)i - toff[dg];
         kout[pos] = key;
         vout[pos] = sval[i];
         // last pass: the sorted position of each input point, into its packed row
@@ -346,6 +377,17 @@ __global__ void k_cell_start(const uint32_t* __restrict__ keys, int64_t n, int64
 
 static inline int check(const char*) { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
+// Keys per counting-sort tile for a sort of n keys: 8192 from kSortLargeMin
+// keys on (HALOPAIR_SORT_TILE=2048 or 8192 forces one size, for tests and
+// measurements).
+static int sort_tile(int64_t n) {
+    if (const char* e = std::getenv("HALOPAIR_SORT_TILE")) {
+        const int v = std::atoi(e);
+        if (v == kSortTile || v == kSortTileLarge) return v;
+    }
+    return n >= kSortLargeMin ? kSortTileLarge : kSortTile;
+}
+
 int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, const double* x, const double* y,
                       const double* z, const double* w, int nw, int64_t n, int64_t total_cells, int key_bits,
                       SortBufs sb, SortedSide out, int32_t* status, hipStream_t s, PackedRows rows,
@@ -359,22 +401,33 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, c
     if (!keys_in)
         hipLaunchKernelGGL(k_keys, dim3(nb), dim3(256), 0, s, geom, nmocks, side, (int)order, x, y, z, n, sb.keys[0],
                            status, order == kHilbert ? keys_row_out : (uint32_t*)nullptr, w, nw, rows);
-    const int ntiles = (int)((n + kSortTile - 1) / kSortTile);
+    const int tile = sort_tile(n);
+    const int ntiles = (int)((n + tile - 1) / tile);
     const int passes = key_bits <= 0 ? 1 : (key_bits + kRadixBits - 1) / kRadixBits;
     int cur = 0;
     for (int pass = 0; pass < passes; pass++) {
         const int shift = pass * kRadixBits;
         const uint32_t* kin = (pass == 0 && keys_in) ? keys_in : sb.keys[cur];
-        hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(256), 0, s, kin, n, shift, sb.tile_counts,
-                           ntiles);
+        if (tile == kSortTileLarge)
+            hipLaunchKernelGGL(k_radix_hist<kSortTileLarge>, dim3(ntiles), dim3(256), 0, s, kin, n, shift,
+                               sb.tile_counts, ntiles);
+        else
+            hipLaunchKernelGGL(k_radix_hist<kSortTile>, dim3(ntiles), dim3(256), 0, s, kin, n, shift,
+                               sb.tile_counts, ntiles);
         hipLaunchKernelGGL(k_radix_scan, dim3(kRadixDigits), dim3(256), 0, s, sb.tile_counts, ntiles,
                            sb.digit_totals);
         // the last pass writes the permutation straight into out.perm
         uint32_t* vdst = (pass == passes - 1) ? out.perm : sb.vals[cur ^ 1];
-        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(256), 0, s, kin,
-                           pass == 0 ? (const uint32_t*)nullptr : sb.vals[cur], sb.keys[cur ^ 1], vdst, n,
-                           shift, sb.tile_counts, sb.digit_totals, ntiles,
-                           (pass == passes - 1 && hpos_to_rows) ? rows.p : (double*)nullptr, rows.stride);
+        const uint32_t* vsrc = pass == 0 ? (const uint32_t*)nullptr : sb.vals[cur];
+        double* inv = (pass == passes - 1 && hpos_to_rows) ? rows.p : (double*)nullptr;
+        if (tile == kSortTileLarge)
+            hipLaunchKernelGGL((k_radix_scatter<kSortTileLarge, 8>), dim3(ntiles), dim3(512), 0, s, kin, vsrc,
+                               sb.keys[cur ^ 1], vdst, n, shift, sb.tile_counts, sb.digit_totals, ntiles, inv,
+                               rows.stride);
+        else
+            hipLaunchKernelGGL((k_radix_scatter<kSortTile, 4>), dim3(ntiles), dim3(256), 0, s, kin, vsrc,
+                               sb.keys[cur ^ 1], vdst, n, shift, sb.tile_counts, sb.digit_totals, ntiles, inv,
+                               rows.stride);
         cur ^= 1;
     }
     if (check("sort")) return -1;

@@ -14,7 +14,9 @@ constexpr int kGroup = 64;           // lane-side points per wave work item (Hil
 constexpr int kTableMax = 512;       // bucket table entries per axis
 constexpr int kRadixBits = 8;        // digit width of the LSD counting sort
 constexpr int kRadixDigits = 1 << kRadixBits;
-constexpr int kSortTile = 2048;      // keys per counting-sort tile (one wave)
+constexpr int kSortTile = 2048;      // keys per counting-sort tile (one workgroup of 4 waves)
+constexpr int kSortTileLarge = 8192; // ... for sorts of at least kSortLargeMin keys (8 waves)
+constexpr int64_t kSortLargeMin = 4 << 20;
 
 // Per-mock geometry, resident in the workspace.  "side" 0 is sample 1, side 1
 // is sample 2.  Cells of the GPU mesh are shared by both sides.
@@ -35,6 +37,8 @@ struct alignas(64) MockGeom {
     float eps;            // absolute safety margin of the f32 window planning
     float r2xy;           // squared planar search radius (with margin)
     int32_t shape;        // 0: sphere (3-D), 1: cylinder (rp, pi)
+    double inv_cell[3];   // 1 / cell: the key pass multiplies (the windows carry a margin)
+    double inv_hcell[3];  // 2^mbits / period: cells per unit length of the lane-ordering grid
 };
 
 // Bucketed bin lookup for an ascending edge array e[0..K-1], e[K] = +inf.
