@@ -213,6 +213,9 @@ constexpr int kWavesPerSimd =
 // Modes whose workgroups are persistent (measured on MI355X: the FP64-histogram
 // passes gain 2-7 %; the integer count passes lose 8-26 %, their register
 // budget leaving no room for the work loop's state without spilling).
+#ifndef HP_SPREAD
+#define HP_SPREAD 1  // spread lane order for the modes with wave-shared histogram rows
+#endif
 #ifndef HP_QUEUES
 #define HP_QUEUES 8  // work queues of a persistent launch (one per XCD)
 #endif
@@ -238,6 +241,19 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
     // 2-D (rp, pi) binning: one histogram per wave, [Kr + 1][Kp + 1] cells
     // (u32 counts, or FP64 weighted sums), the last row / column being trash
     constexpr bool XY = (MODE == kModeXYZ || MODE == kModeWXYZ || MODE == kModeJackXY);
+    // Modes whose histogram rows are shared by every lane of the wave (the
+    // (rp, pi) histograms, the jackknife loss rows) hand out a full group's
+    // points to the lanes in a spread order: lane l holds the point at
+    // position 4 (l mod 16) + l / 16 of the group's Hilbert run, so the 16 or
+    // 32 lanes that one LDS pass serves hold points from across the whole
+    // group, which fall into different (rp, pi) cells more often than
+    // neighbours along the curve do (fewer lanes of a pass on one address).
+    // Partial groups keep the identity order.  The symmetric self block reads
+    // stream point k as the point lane k holds, so pairs keep their lane order.
+    constexpr bool SPREAD = HP_SPREAD && (XY || MODE == kModeJack);
+    auto spread_pos = [&](int l, int n) __attribute__((always_inline)) {
+        return (SPREAD && n == kGroup) ? (((l & 15) << 2) | (l >> 4)) : l;
+    };
     constexpr uint32_t cell_bytes = (MODE == kModeWXYZ || MODE == kModeJackXY) ? 8u : 4u;
     // LDS: the bucket tables (sized to this request's buckets), then one area per wave
     extern __shared__ __align__(16) unsigned char lds[];
@@ -347,8 +363,10 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
         const int64_t p0 = G.p_off[side] + gl * kGroup;
         const int64_t left = G.n[side] - gl * kGroup;
         const int cnt = uni((int)(left < kGroup ? left : kGroup));
-        const bool act = lane < cnt;
-        const int64_t p = p0 + (act ? lane : 0);
+        // position in the group's Hilbert run of the point this lane holds
+        const int lpos = spread_pos(lane, cnt);
+        const bool act = lpos < cnt;
+        const int64_t p = p0 + (act ? lpos : 0);
         // lane points are read in input order through the Hilbert permutation
         const int64_t q = (int64_t)L.lane.perm[p];
         // inactive lanes carry NaN: every comparison fails, so they land in the trash row
@@ -940,7 +958,8 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                 double bx[B], by[B], bz[B], bw[B], bt[B];
 #pragma unroll
                 for (int u = 0; u < B; u++) {
-                    const int64_t qk = (int64_t)sload_u32_at(L.lane.perm, p0 + min(kb + u, cnt - 1));
+                    const int64_t qk =
+                        (int64_t)sload_u32_at(L.lane.perm, p0 + spread_pos(min(kb + u, cnt - 1), cnt));
                     bx[u] = sload_f64_at(L.lane.x, qk);
                     by[u] = sload_f64_at(L.lane.y, qk);
                     bz[u] = sload_f64_at(L.lane.z, qk);
