@@ -134,7 +134,10 @@ struct Plan {
     int64_t ngroups[2] = {0, 0};
     int64_t max_groups[2] = {0, 0};
     int64_t total_cells = 0, total_keys = 0;
-    int bits_row = 1, bits_hilbert = 1;
+    int bits_row = 1, bits_hilbert = 1;  // key bits of the largest mock (keys are local to a mock)
+    int tile[2] = {0, 0}, ntiles[2] = {0, 0};  // sort tiles of each side
+    std::vector<uint32_t> tfirst[2];           // first tile of each mock, then the tile count
+    size_t o_tfirst[2] = {0, 0};
     std::vector<MockGeom> geom;
     Table tr, tp;
     int Kr = 0, Kp = 0, nb_count = 0;
@@ -260,6 +263,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     // ---- per-mock geometry
     P.geom.resize((size_t)P.M);
     int64_t p_off[2] = {0, 0}, g_off[2] = {0, 0}, cell_off = 0, key_off = 0;
+    int64_t max_cells = 1, max_hkeys = 1;
     for (int m = 0; m < P.M; m++) {
         const hp_mock& hm = r->mocks[m];
         MockGeom& G = P.geom[(size_t)m];
@@ -322,6 +326,8 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
         G.key_off = key_off;
         cell_off += ncells;
         key_off += (int64_t)1 << (3 * mb);
+        max_cells = std::max<int64_t>(max_cells, (int64_t)ncells);
+        max_hkeys = std::max<int64_t>(max_hkeys, (int64_t)1 << (3 * mb));
         if (nm[1] > 0 && (double)nm[1] * 27.0 >= 4.0e9) return fail("too many points in one mock for 32-bit histograms");
         // the pair kernel addresses a mock's stream points (and weights) with 32-bit byte offsets
         for (int s = 0; s < 2; s++)
@@ -336,8 +342,21 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     P.total_cells = cell_off;
     P.total_keys = key_off;
     if (cell_off >= (int64_t)0xffffffff || key_off > (int64_t)0xffffffff) return fail("mesh too large for 32-bit keys");
-    P.bits_row = bits_for((uint64_t)cell_off);
-    P.bits_hilbert = bits_for((uint64_t)key_off);
+    P.bits_row = bits_for((uint64_t)max_cells);
+    P.bits_hilbert = bits_for((uint64_t)max_hkeys);
+    // sort tiles per side: a mock's points start a new tile
+    for (int s = 0; s < 2; s++) {
+        P.tile[s] = sort_tile(P.n[s]);
+        P.tfirst[s].assign((size_t)P.M + 1, 0u);
+        int64_t t = 0;
+        for (int m = 0; m < P.M; m++) {
+            P.tfirst[s][m] = (uint32_t)t;
+            t += (P.geom[m].n[s] + P.tile[s] - 1) / P.tile[s];
+        }
+        P.tfirst[s][P.M] = (uint32_t)t;
+        if (t >= (int64_t)0x7fffffff) return fail("too many sort tiles");
+        P.ntiles[s] = (int)std::max<int64_t>(t, 1);
+    }
 
     // ---- workspace layout
     size_t off = 0;
@@ -351,6 +370,8 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     P.o_ep = take(P.tp.e.size() * 8);
     P.o_tr = take(P.tr.t.size() * sizeof(TabEntry));
     P.o_tp = take(P.tp.t.size() * sizeof(TabEntry));
+    P.o_tfirst[0] = take(((size_t)P.M + 1) * 4);
+    P.o_tfirst[1] = take(((size_t)P.M + 1) * 4);
     const size_t consts_end = off;
     P.o_status = take(16);
     P.o_gm[0] = take((size_t)P.ngroups[0] * 4 + 4);
@@ -381,7 +402,8 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
             if (o == kRowMajor) S.cs = take((size_t)(P.total_cells + 1) * 4);
         }
     }
-    const int64_t ntiles = (nmax + kSortTile - 1) / kSortTile;
+    // digit counts per tile: room for the smaller tiles whichever size runs
+    const int64_t ntiles = (nmax + kSortTile - 1) / kSortTile + P.M + 1;
     P.o_keys[0] = take((size_t)nmax * 4);
     P.o_keys_row = take((size_t)nmax * 4);
     P.o_keys[1] = take((size_t)nmax * 4);
@@ -443,6 +465,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     std::memcpy(P.consts.data() + P.o_ep, P.tp.e.data(), P.tp.e.size() * 8);
     std::memcpy(P.consts.data() + P.o_tr, P.tr.t.data(), P.tr.t.size() * sizeof(TabEntry));
     std::memcpy(P.consts.data() + P.o_tp, P.tp.t.data(), P.tp.t.size() * sizeof(TabEntry));
+    for (int s = 0; s < 2; s++) std::memcpy(P.consts.data() + P.o_tfirst[s], P.tfirst[s].data(), P.tfirst[s].size() * 4);
     return 0;
 }
 
@@ -536,8 +559,12 @@ static int run(const hp_request* r, int kind) {
                 rows.nw = P.row_nw[sd];
             }
             static_assert(kHilbert < kRowMajor, "the Hilbert order must be built before the row-major order");
+            SortTiles st;
+            st.first = reinterpret_cast<const uint32_t*>(ws + P.o_tfirst[sd]);
+            st.tile = P.tile[sd];
+            st.ntiles = P.ntiles[sd];
             if (build_sorted_side(dgeom, P.M, sd, (Order)o, X[sd], Y[sd], Z[sd], P.weighted() ? W[sd] : nullptr, P.nw,
-                                  P.n[sd], P.total_cells, o == kRowMajor ? P.bits_row : P.bits_hilbert, sb,
+                                  P.n[sd], P.total_cells, o == kRowMajor ? P.bits_row : P.bits_hilbert, sb, st,
                                   side_ptrs(ws, S), status, s, rows, rank && o == kHilbert,
                                   o == kRowMajor ? hrank : nullptr, o == kHilbert ? krow : nullptr,
                                   o == kRowMajor ? krow : nullptr))

@@ -10,8 +10,8 @@
 //              column are contiguous, so a column's z-range is one segment;
 //   Hilbert    3-D Hilbert index    -- lane side; 64 consecutive points form a
 //              spatially compact group (the curve has no jumps).
-// A key of mock m is offset by that mock's first cell (row-major) or first
-// Hilbert index, so one sort handles a whole batch of realisations.
+// Keys are local to a mock; the sort's tiles keep mocks apart, so one sort
+// handles a whole batch of realisations with the key bits of one mock.
 #include <cstdlib>
 #include <cstring>
 
@@ -131,8 +131,9 @@ __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, 
     const int ix = cell_index(px, G.inv_cell[0], G.ndiv[0], bad, G.period[0], G.pbc);
     const int iy = cell_index(py, G.inv_cell[1], G.ndiv[1], bad, G.period[1], G.pbc);
     const int iz = cell_index(pz, G.inv_cell[2], G.ndiv[2], bad, G.period[2], G.pbc);
+    // keys are local to the mock: the sort keeps mocks apart by their tiles
     uint32_t key;
-    const uint32_t rkey = (uint32_t)(G.cell_off + ((int64_t)ix * G.ndiv[1] + iy) * G.ndiv[2] + iz);
+    const uint32_t rkey = (uint32_t)(((int64_t)ix * G.ndiv[1] + iy) * G.ndiv[2] + iz);
     if (order == kRowMajor) {
         key = rkey;
     } else {
@@ -145,7 +146,7 @@ __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, 
         const int hx = cell_index(px, G.inv_hcell[0], hn, dummy, G.period[0], 0);
         const int hy = cell_index(py, G.inv_hcell[1], hn, dummy, G.period[1], 0);
         const int hz = cell_index(pz, G.inv_hcell[2], hn, dummy, G.period[2], 0);
-        key = (uint32_t)(G.key_off + hilbert3((uint32_t)hx, (uint32_t)hy, (uint32_t)hz, hb));
+        key = hilbert3((uint32_t)hx, (uint32_t)hy, (uint32_t)hz, hb);
     }
     keys[p] = key;
     if (bad && status) atomicOr(status, 1);
@@ -158,16 +159,44 @@ __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, 
 // 256 scattered words per tile, and a tile's keys leave in 256 runs, so 8192
 // keys per tile (runs of ~32 keys, full cache lines) move a fraction of the
 // bytes of 2048-key tiles (runs of ~8 keys) at tens of millions of points.
+//
+// Mocks are sorted side by side: the points of a mock are contiguous, its
+// tiles never cross into the next mock (tile_first[m] = first tile of mock m,
+// tile_first[nmocks] = tiles in all), and a mock's keys go to its own range of
+// positions, so the keys only need the bits of the largest mock (one or two
+// passes fewer than keys offset by mock for a batch of mocks).
+struct TileSpan {
+    int64_t start;  // first key of the tile
+    int tn;         // keys in the tile
+    int m;          // its mock
+};
+
+__device__ __forceinline__ TileSpan tile_span(const uint32_t* __restrict__ tile_first, const MockGeom* __restrict__ geom,
+                                              int nmocks, int side, int t, int tile) {
+    int lo = 0, hi = nmocks - 1;  // the last mock whose first tile is <= t (skips empty mocks)
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)tile_first[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    TileSpan r;
+    r.m = lo;
+    const int64_t k = (int64_t)(t - (int)tile_first[lo]) * tile;
+    r.start = geom[lo].p_off[side] + k;
+    r.tn = (int)min((int64_t)tile, geom[lo].n[side] - k);
+    return r;
+}
+
 template <int TILE>
-__global__ void __launch_bounds__(256) k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
-                                                    uint32_t* __restrict__ tile_counts, int ntiles) {
+__global__ void __launch_bounds__(256) k_radix_hist(const uint32_t* __restrict__ keys, const MockGeom* __restrict__ geom,
+                                                    const uint32_t* __restrict__ tile_first, int nmocks, int side,
+                                                    int shift, uint32_t* __restrict__ tile_counts, int ntiles) {
     __shared__ uint32_t h[kRadixDigits];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * TILE;
-    if (base + TILE <= n) {
-        // full tile: 16-byte loads
-        const uint4* k4 = reinterpret_cast<const uint4*>(keys + base);
+    const TileSpan ts = tile_span(tile_first, geom, nmocks, side, (int)blockIdx.x, TILE);
+    if (ts.tn == TILE && (ts.start & 3) == 0) {
+        // full, aligned tile: 16-byte loads
+        const uint4* k4 = reinterpret_cast<const uint4*>(keys + ts.start);
         for (int i = threadIdx.x; i < TILE / 4; i += 256) {
             const uint4 k = k4[i];
             atomicAdd(&h[(k.x >> shift) & (kRadixDigits - 1)], 1u);
@@ -176,10 +205,8 @@ __global__ void __launch_bounds__(256) k_radix_hist(const uint32_t* __restrict__
             atomicAdd(&h[(k.w >> shift) & (kRadixDigits - 1)], 1u);
         }
     } else {
-        for (int i = threadIdx.x; i < TILE; i += 256) {
-            const int64_t idx = base + i;
-            if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (kRadixDigits - 1)], 1u);
-        }
+        for (int i = threadIdx.x; i < ts.tn; i += 256)
+            atomicAdd(&h[(keys[ts.start + i] >> shift) & (kRadixDigits - 1)], 1u);
     }
     __syncthreads();
     tile_counts[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
@@ -234,11 +261,18 @@ __device__ __forceinline__ uint32_t scan_digits(uint32_t* scan, uint32_t v) {
 // comes from eight ballots; per-wave digit counters carry it across rounds),
 // the tile is staged in LDS in digit order, and the staged keys go out in
 // runs of equal digits, so consecutive threads write consecutive addresses.
+// A mock's keys go to its own positions: position p_off[m] + (keys of the
+// mock with smaller digits) + (keys of its earlier tiles with the same digit)
+// + rank in the tile, all from the exclusive prefix G[d][t] of the digit
+// counts over every tile.  The last row-major pass writes global cell ids
+// (key + the mock's first cell) for the cell-start table.
 template <int TILE, int WAVES>
 __global__ void __launch_bounds__(WAVES * 64) k_radix_scatter(const uint32_t* __restrict__ kin,
                                                               const uint32_t* __restrict__ vin,
                                                               uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                              int64_t n, int shift,
+                                                              const MockGeom* __restrict__ geom,
+                                                              const uint32_t* __restrict__ tile_first, int nmocks,
+                                                              int side, int shift, int add_cell_off,
                                                               const uint32_t* __restrict__ tile_counts,
                                                               const uint32_t* __restrict__ digit_totals, int ntiles,
                                                               double* __restrict__ inv_rows, int inv_stride) {
@@ -256,8 +290,10 @@ __global__ void __launch_bounds__(WAVES * 64) k_radix_scatter(const uint32_t* __
     const int lane = tid & 63;
     const int wv = tid >> 6;
     const int tile = blockIdx.x;
-    const int64_t base = (int64_t)tile * TILE;
-    const int tn = (int)min((int64_t)TILE, n - base);
+    const TileSpan ts = tile_span(tile_first, geom, nmocks, side, tile, TILE);
+    const int64_t base = ts.start;
+    const int tn = ts.tn;
+    const uint32_t koff = add_cell_off ? (uint32_t)geom[ts.m].cell_off : 0u;
 
     for (int i = tid; i < WAVES * kRadixDigits; i += NT) (&wcnt[0][0])[i] = 0u;
     __syncthreads();
@@ -306,12 +342,20 @@ __global__ void __launch_bounds__(WAVES * 64) k_radix_scatter(const uint32_t* __
             run += c;
         }
     }
-    // global base: all keys of smaller digits (an exclusive scan of the digit
-    // totals, the same for every tile), then this digit's keys of earlier tiles
-    const uint32_t t = d < kRadixDigits ? digit_totals[d] : 0u;
+    // global base: the mock's first position, the mock's keys of smaller
+    // digits (an exclusive scan of the mock's digit totals), then this digit's
+    // keys of the mock's earlier tiles
+    const int f0 = (int)tile_first[ts.m], f1 = (int)tile_first[ts.m + 1];
+    uint32_t t = 0, g0 = 0;
+    if (d < kRadixDigits) {
+        const uint32_t* row = tile_counts + (int64_t)d * ntiles;
+        g0 = row[f0];
+        t = (f1 < ntiles ? row[f1] : digit_totals[d]) - g0;
+    }
     __syncthreads();
     const uint32_t tincl = scan_digits<NT>(scan, t);
-    if (d < kRadixDigits) gbase[d] = tile_counts[(int64_t)d * ntiles + tile] + tincl - t;
+    if (d < kRadixDigits)
+        gbase[d] = (uint32_t)geom[ts.m].p_off[side] + tincl - t + (tile_counts[(int64_t)d * ntiles + tile] - g0);
     __syncthreads();
 
     // ---- stage in digit order
@@ -330,7 +374,7 @@ __global__ void __launch_bounds__(WAVES * 64) k_radix_scatter(const uint32_t* __
         const uint32_t key = skey[i];
         const uint32_t dg = (key >> shift) & (kRadixDigits - 1);
         const uint32_t pos = gbase[dg] + (uint32_t)i - toff[dg];
-        kout[pos] = key;
+        kout[pos] = key + koff;
         vout[pos] = sval[i];
         // last pass: the sorted position of each input point, into its packed row
         if (inv_rows) reinterpret_cast<uint32_t*>(inv_rows + (int64_t)sval[i] * inv_stride)[6] = pos;
@@ -380,7 +424,7 @@ static inline int check(const char*) { return hipGetLastError() == hipSuccess ? 
 // Keys per counting-sort tile for a sort of n keys: 8192 from kSortLargeMin
 // keys on (HALOPAIR_SORT_TILE=2048 or 8192 forces one size, for tests and
 // measurements).
-static int sort_tile(int64_t n) {
+int sort_tile(int64_t n) {
     if (const char* e = std::getenv("HALOPAIR_SORT_TILE")) {
         const int v = std::atoi(e);
         if (v == kSortTile || v == kSortTileLarge) return v;
@@ -390,7 +434,7 @@ static int sort_tile(int64_t n) {
 
 int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, const double* x, const double* y,
                       const double* z, const double* w, int nw, int64_t n, int64_t total_cells, int key_bits,
-                      SortBufs sb, SortedSide out, int32_t* status, hipStream_t s, PackedRows rows,
+                      SortBufs sb, SortTiles st, SortedSide out, int32_t* status, hipStream_t s, PackedRows rows,
                       bool hpos_to_rows, uint32_t* hrank_out, uint32_t* keys_row_out, const uint32_t* keys_in) {
     if (n <= 0) {
         if (order == kRowMajor) return launch_fill_u32(out.cell_start, 0u, total_cells + 1, s);
@@ -401,33 +445,37 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, c
     if (!keys_in)
         hipLaunchKernelGGL(k_keys, dim3(nb), dim3(256), 0, s, geom, nmocks, side, (int)order, x, y, z, n, sb.keys[0],
                            status, order == kHilbert ? keys_row_out : (uint32_t*)nullptr, w, nw, rows);
-    const int tile = sort_tile(n);
-    const int ntiles = (int)((n + tile - 1) / tile);
+    const int tile = st.tile, ntiles = st.ntiles;
+    if (tile != kSortTile && tile != kSortTileLarge) return -1;
+    if (ntiles <= 0) return -1;
     const int passes = key_bits <= 0 ? 1 : (key_bits + kRadixBits - 1) / kRadixBits;
     int cur = 0;
     for (int pass = 0; pass < passes; pass++) {
         const int shift = pass * kRadixBits;
         const uint32_t* kin = (pass == 0 && keys_in) ? keys_in : sb.keys[cur];
         if (tile == kSortTileLarge)
-            hipLaunchKernelGGL(k_radix_hist<kSortTileLarge>, dim3(ntiles), dim3(256), 0, s, kin, n, shift,
-                               sb.tile_counts, ntiles);
+            hipLaunchKernelGGL(k_radix_hist<kSortTileLarge>, dim3(ntiles), dim3(256), 0, s, kin, geom, st.first,
+                               nmocks, side, shift, sb.tile_counts, ntiles);
         else
-            hipLaunchKernelGGL(k_radix_hist<kSortTile>, dim3(ntiles), dim3(256), 0, s, kin, n, shift,
-                               sb.tile_counts, ntiles);
+            hipLaunchKernelGGL(k_radix_hist<kSortTile>, dim3(ntiles), dim3(256), 0, s, kin, geom, st.first, nmocks,
+                               side, shift, sb.tile_counts, ntiles);
         hipLaunchKernelGGL(k_radix_scan, dim3(kRadixDigits), dim3(256), 0, s, sb.tile_counts, ntiles,
                            sb.digit_totals);
-        // the last pass writes the permutation straight into out.perm
-        uint32_t* vdst = (pass == passes - 1) ? out.perm : sb.vals[cur ^ 1];
+        // the last pass writes the permutation straight into out.perm, and
+        // (row-major) the global cell ids for the cell-start table
+        const bool last = pass == passes - 1;
+        uint32_t* vdst = last ? out.perm : sb.vals[cur ^ 1];
         const uint32_t* vsrc = pass == 0 ? (const uint32_t*)nullptr : sb.vals[cur];
-        double* inv = (pass == passes - 1 && hpos_to_rows) ? rows.p : (double*)nullptr;
+        double* inv = (last && hpos_to_rows) ? rows.p : (double*)nullptr;
+        const int add_off = (last && order == kRowMajor) ? 1 : 0;
         if (tile == kSortTileLarge)
             hipLaunchKernelGGL((k_radix_scatter<kSortTileLarge, 8>), dim3(ntiles), dim3(512), 0, s, kin, vsrc,
-                               sb.keys[cur ^ 1], vdst, n, shift, sb.tile_counts, sb.digit_totals, ntiles, inv,
-                               rows.stride);
+                               sb.keys[cur ^ 1], vdst, geom, st.first, nmocks, side, shift, add_off, sb.tile_counts,
+                               sb.digit_totals, ntiles, inv, rows.stride);
         else
             hipLaunchKernelGGL((k_radix_scatter<kSortTile, 4>), dim3(ntiles), dim3(256), 0, s, kin, vsrc,
-                               sb.keys[cur ^ 1], vdst, n, shift, sb.tile_counts, sb.digit_totals, ntiles, inv,
-                               rows.stride);
+                               sb.keys[cur ^ 1], vdst, geom, st.first, nmocks, side, shift, add_off, sb.tile_counts,
+                               sb.digit_totals, ntiles, inv, rows.stride);
         cur ^= 1;
     }
     if (check("sort")) return -1;

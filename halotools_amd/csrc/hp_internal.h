@@ -72,6 +72,13 @@ struct SortBufs {
     uint32_t* digit_totals;  // [digits]
 };
 
+// Tiles of one side's sort: tiles never cross from one mock into the next.
+struct SortTiles {
+    const uint32_t* first = nullptr;  // [nmocks + 1]: first tile of each mock, then the tile count
+    int tile = 0;                     // keys per tile (kSortTile or kSortTileLarge)
+    int ntiles = 0;
+};
+
 enum Order : int { kHilbert = 0, kRowMajor = 1 };
 
 // Sorted copy of one side in one order.
@@ -112,10 +119,13 @@ int launch_group_mock(const MockGeom* g, int nmocks, int side, int32_t* group_mo
 // Hilbert build also writes the row-major keys (keys_row_out) and the
 // row-major build starts from them (keys_in).  Null pointers skip each of
 // these.
+// Keys per sort tile for a sort of n keys (HALOPAIR_SORT_TILE = 2048 or 8192
+// forces one size, for tests and measurements).
+int sort_tile(int64_t n);
 int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order,
                       const double* x, const double* y, const double* z, const double* w,
                       int nw, int64_t n, int64_t total_cells, int key_bits,
-                      SortBufs sb, SortedSide out, int32_t* status, hipStream_t s, PackedRows rows,
+                      SortBufs sb, SortTiles st, SortedSide out, int32_t* status, hipStream_t s, PackedRows rows,
                       bool hpos_to_rows = false, uint32_t* hrank_out = nullptr, uint32_t* keys_row_out = nullptr,
                       const uint32_t* keys_in = nullptr);
 
