@@ -29,7 +29,7 @@ ALG_BUILD_BYTES = 76
 # HBM bytes per launch of the pair kernel on this workload: rocprofv3
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors being what the same
 # counters read for a plain 256 MiB copy (profiles/k_pairs_count3d_clustered_1M.md)
-PAIR_KERNEL_TRAFFIC_BYTES = 4.097e8
+PAIR_KERNEL_TRAFFIC_BYTES = 1.6792e9
 
 
 def _progress(msg):
@@ -97,14 +97,7 @@ def bench_batched(dev, steps):
     t_tune = time.perf_counter() - t0
     _progress("batched: tuned in %.1f s" % t_tune)
 
-    # cell build at this scale (all 64 mocks in one counting sort): HIP events
-    build_ms = []
-    for _ in range(3):
-        ev = _events(torch)
-        b.npairs_3d(build_events=(ev[0].cuda_event, ev[1].cuda_event))
-        torch.cuda.synchronize()
-        build_ms.append(ev[0].elapsed_time(ev[1]))
-    build_s = float(np.median(build_ms)) * 1e-3
+    build_s = _time_batch_build(b, 3)
 
     lm_all = torch.cat(lm)
 
@@ -459,6 +452,36 @@ def _events(torch):
     return ev
 
 
+def _time_batch_build(b, reps):
+    """Cell build of a MockBatch (all mocks in one counting sort): HIP events
+    around the build of an npairs_3d call; median of reps calls, in seconds."""
+    import numpy as np
+    import torch
+    build_ms = []
+    for _ in range(reps):
+        ev = _events(torch)
+        b.npairs_3d(build_events=(ev[0].cuda_event, ev[1].cuda_event))
+        torch.cuda.synchronize()
+        build_ms.append(ev[0].elapsed_time(ev[1]))
+    return float(np.median(build_ms)) * 1e-3
+
+
+def bench_build64(dev, steps):
+    """The 64-mock batch's cell build alone (the batched leg's mocks and tuned
+    cells, without the steps): 37.2M points sorted in both orders."""
+    import torch
+
+    from halotools_amd import autotune, batch, mocks
+    sizes, pts_h, _, _, _, L = mocks.many_mocks()
+    pts = [torch.tensor(p, device=dev) for p in pts_h]
+    del pts_h
+    b = batch.MockBatch(pts, L, mocks.config_rbins(5), device=dev)
+    b.tune(autotune.TANGENT, reps=1)
+    build_s = _time_batch_build(b, max(3, steps))
+    _progress("build64: %.3f ms" % (1e3 * build_s))
+    return _build_rate(int(sizes.sum()), build_s)
+
+
 def _build_rate(npts, seconds):
     """Cell build of one auto-count sample against the HBM roofline.
 
@@ -482,7 +505,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-mock batched leg")
     ap.add_argument("--no-others", action="store_true", help="skip the xy_z / projected / marked timings")
     ap.add_argument("--legs", default="all",
-                    help="comma list of legs to run besides the headline: batched, " + ", ".join(OTHER_LEGS))
+                    help="comma list of legs to run besides the headline: batched, " + ", ".join(OTHER_LEGS) +
+                         "; build64 (the batch's cell build alone) runs only when named")
     return ap.parse_args()
 
 
@@ -612,6 +636,7 @@ def main():
 
     legs = set(["batched"] + list(OTHER_LEGS)) if args.legs == "all" else set(args.legs.split(","))
     batched = None if (args.no_batched or "batched" not in legs) else bench_batched(dev, max(3, args.steps // 4))
+    build64 = bench_build64(dev, max(3, args.steps // 4)) if "build64" in legs else None
     other = [g for g in OTHER_LEGS if g in legs]
     others = None if (args.no_others or not other) else bench_other_configs(
         dev, max(3, args.steps // 4), cpu_threads=0 if args.no_cpu_baseline else threads, legs=other)
@@ -648,6 +673,7 @@ def main():
                                        "workload, scaled by a plain copy's readings "
                                        "(profiles/k_pairs_count3d_clustered_1M.md)"},
         "batched_64_mocks": batched,
+        "cell_build_64_mocks": build64,
         "other_configs": others,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,

@@ -57,8 +57,17 @@ def _max(t):
     return int(t.max()) if isinstance(t, torch.Tensor) else int(np.max(t))
 
 
-def _unique(t):
-    return torch.unique(t).cpu().numpy() if isinstance(t, torch.Tensor) else np.unique(t)
+def _covers_all(t, N_samples):
+    """np.array_equal(np.unique(t), np.arange(1, N_samples + 1)) for tags
+    already checked to lie in [1, N_samples]: every tag value occurs.  A
+    histogram instead of a sort (no more bins than points: with fewer points
+    than subsamples some tag is missing)."""
+    n = _n(t)
+    if N_samples > n:
+        return False
+    if isinstance(t, torch.Tensor):
+        return bool(torch.bincount(t.reshape(-1), minlength=N_samples + 1)[1:].min() > 0)
+    return bool(np.bincount(np.ravel(t), minlength=N_samples + 1)[1:].min() > 0)
 
 
 def _process_weights_jtags(sample1, sample2, weights1, weights2, jtags1, jtags2, N_samples):
@@ -75,10 +84,11 @@ def _process_weights_jtags(sample1, sample2, weights1, weights2, jtags1, jtags2,
         raise HalotoolsError("jtags1 must be <= N_samples")
     if _max(jtags2) > N_samples:
         raise HalotoolsError("jtags2 must be <= N_samples")
-    if not np.array_equal(_unique(jtags1), np.arange(1, N_samples + 1)):
+    covers = _covers_all(jtags1, N_samples)
+    if not covers:
         warn("Warning: sample1 does not contain points in every jackknife sample.")
     # the reference tests jtags1 again for its sample2 warning (PC/npairs_jackknife_3d.py:273-274)
-    if not np.array_equal(_unique(jtags1), np.arange(1, N_samples + 1)):
+    if not covers:
         warn("Warning: sample2 does not contain points in every jackknife sample.")
     return weights1, weights2, jtags1, jtags2
 

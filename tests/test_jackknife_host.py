@@ -115,3 +115,21 @@ def test_drop_in_argument_errors_before_any_gpu_work():
         npairs_jackknife_3d(s1, s2, np.array([0.1, 0.5]), t1, t2, 5, period=1.0)
     with pytest.raises(ValueError):
         npairs_jackknife_xy_z(s1, s2, np.array([0.1, 0.2]), np.array([0.1, 0.5]), t1, t2, 5, period=1.0)
+
+
+def test_tag_coverage_check_matches_unique():
+    """The coverage check behind the reference's warnings
+    (np.array_equal(np.unique(jtags), np.arange(1, N + 1)),
+    PC/npairs_jackknife_3d.py:270-274) is a histogram here: same answer on
+    NumPy and torch tags, including fewer points than subsamples."""
+    import torch
+    from halotools_amd.mock_observables.pair_counters.npairs_jackknife_3d import _covers_all
+    rng = np.random.default_rng(5)
+    for n, N in [(1, 1), (3, 5), (10, 10), (50, 7), (200, 64), (40, 40)]:
+        for _ in range(20):
+            t = rng.integers(1, N + 1, n)
+            want = np.array_equal(np.unique(t), np.arange(1, N + 1))
+            assert _covers_all(t, N) == want
+            assert _covers_all(torch.from_numpy(t), N) == want
+    full = np.arange(1, 9)
+    assert _covers_all(full, 8) and not _covers_all(full[:-1], 8)
