@@ -82,14 +82,18 @@ def _marked_npairs_process_weights(sample1, sample2, weights1, weights2, weight_
 
 
 def same_weights(w1, w2):
-    """True when both weight arrays hold the same values (NumPy or device tensors)."""
+    """True when both weight arguments are known to hold the same values: the
+    same object, the same view of one device tensor (device tensors are not
+    compared by value, which would wait for the device), or equal host arrays.
+    A False only costs the symmetric evaluation of an auto-count."""
     if w1 is w2:
         return True
     if isinstance(w1, torch.Tensor) or isinstance(w2, torch.Tensor):
-        if not (isinstance(w1, torch.Tensor) and isinstance(w2, torch.Tensor)) or w1.device != w2.device:
-            return False
-        return w1.shape == w2.shape and bool(torch.equal(w1, w2))
-    return np.array_equal(w1, w2)
+        return D.same_array(w1, w2)
+    try:
+        return bool(np.array_equal(w1, w2))
+    except (TypeError, ValueError):
+        return False
 
 
 def marked_npairs_3d(sample1, sample2, rbins, weight_func_id,
@@ -121,13 +125,14 @@ def marked_npairs_3d(sample1, sample2, rbins, weight_func_id,
     approx_cell1_size, approx_cell2_size = P.default_cell_sizes(
         approx_cell1_size, approx_cell2_size, (rmax, rmax, rmax))
     xperiod, yperiod, zperiod = period
+    raw_same = same_weights(weights1, weights2)
     weights1, weights2 = _marked_npairs_process_weights(sample1, sample2, weights1, weights2, weight_func_id)
     P.set_approximate_cell_sizes(approx_cell1_size, approx_cell2_size, period)
     P.check_search_lengths((rmax, rmax, rmax), (xperiod, yperiod, zperiod))
 
     device = D.require_gpu(sample1.device if D.is_device_tensor(sample1) else None)
     as_tensor = D.is_device_tensor(sample1)
-    same = D.same_array(sample1, sample2) and same_weights(weights1, weights2)
+    same = D.same_array(sample1, sample2) and (raw_same or same_weights(weights1, weights2))
     s1, s2 = C.upload((x1, y1, z1), (x2, y2, z2), D.same_array(sample1, sample2), device)
     nw = weights1.shape[1]
     w1 = D.to_device_weights(weights1, device)

@@ -60,6 +60,7 @@ def marked_npairs_xy_z(sample1, sample2, rp_bins, pi_bins,
         approx_cell1_size, approx_cell2_size, (rp_max, rp_max, pi_max))
     xperiod, yperiod, zperiod = period
     # PC/marked_npairs_xy_z.py:161-164, then the mesh checks of :166-175
+    raw_same = same_weights(weights1, weights2)
     weights1, weights2 = _marked_npairs_process_weights(sample1, sample2, weights1, weights2, weight_func_id,
                                                         signature=_func_signature_int_from_wfunc)
     P.set_approximate_cell_sizes(approx_cell1_size, approx_cell2_size, period)
@@ -67,7 +68,7 @@ def marked_npairs_xy_z(sample1, sample2, rp_bins, pi_bins,
 
     device = D.require_gpu(sample1.device if D.is_device_tensor(sample1) else None)
     as_tensor = D.is_device_tensor(sample1)
-    same = D.same_array(sample1, sample2) and same_weights(weights1, weights2)
+    same = D.same_array(sample1, sample2) and (raw_same or same_weights(weights1, weights2))
     s1, s2 = C.upload((x1, y1, z1), (x2, y2, z2), D.same_array(sample1, sample2), device)
     nw = weights1.shape[1]
     w1 = D.to_device_weights(weights1, device)

@@ -99,24 +99,27 @@ _npairs_jackknife_xy_z_process_weights_jtags = _process_weights_jtags
 
 
 def _same_values(a, b):
+    """Known to hold the same values: one object, one view of a device tensor
+    (not compared by value, which would wait for the device), or equal host
+    arrays.  A False only costs the symmetric evaluation of an auto-count."""
     if a is b:
         return True
     if a is None or b is None:
         return False
     if isinstance(a, torch.Tensor) or isinstance(b, torch.Tensor):
-        if not (isinstance(a, torch.Tensor) and isinstance(b, torch.Tensor)) or a.device != b.device:
-            return False
-        return a.shape == b.shape and bool(torch.equal(a, b))
+        return D.same_array(a, b)
     return np.array_equal(a, b)
 
 
-def _run(kind, sample1, sample2, cols1, cols2, period, pbc, r2, p2, weights1, weights2, jtags1, jtags2, N_samples):
+def _run(kind, sample1, sample2, cols1, cols2, period, pbc, r2, p2, weights1, weights2, jtags1, jtags2, N_samples,
+         raw_same=False):
     if N_samples >= 2 ** 31:
         raise ValueError("N_samples must be below 2**31")
     device = D.require_gpu(sample1.device if D.is_device_tensor(sample1) else None)
     as_tensor = D.is_device_tensor(sample1)
-    same = (D.same_array(sample1, sample2) and _same_values(weights1, weights2)
-            and _same_values(jtags1, jtags2))
+    # raw_same: the caller's weights and tags arguments were already known equal
+    same = D.same_array(sample1, sample2) and (raw_same or (_same_values(weights1, weights2)
+                                                            and _same_values(jtags1, jtags2)))
     s1, s2 = C.upload(cols1, cols2, D.same_array(sample1, sample2), device)
     tag = lambda t: (t if isinstance(t, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(t))).to(
         device=device, dtype=torch.int32).contiguous()
@@ -165,12 +168,13 @@ def npairs_jackknife_3d(sample1, sample2, rbins, jtags1, jtags2, N_samples, peri
     approx_cell1_size, approx_cell2_size = P.default_cell_sizes(
         approx_cell1_size, approx_cell2_size, (rmax, rmax, rmax))
     xperiod, yperiod, zperiod = period
+    raw_same = _same_values(weights1, weights2) and _same_values(jtags1, jtags2)
     weights1, weights2, jtags1, jtags2 = _process_weights_jtags(sample1, sample2, weights1, weights2, jtags1,
                                                                 jtags2, N_samples)
     P.set_approximate_cell_sizes(approx_cell1_size, approx_cell2_size, period)
     P.check_search_lengths((rmax, rmax, rmax), (xperiod, yperiod, zperiod))
     return _run(E.HP_NPAIRS_JACKKNIFE_3D, sample1, sample2, (x1, y1, z1), (x2, y2, z2), period, pbc,
-                P.squared(rbins), None, weights1, weights2, jtags1, jtags2, N_samples)
+                P.squared(rbins), None, weights1, weights2, jtags1, jtags2, N_samples, raw_same)
 
 
 def npairs_jackknife_xy_z(sample1, sample2, rp_bins, pi_bins, jtags1, jtags2, N_samples, period=None,
@@ -198,9 +202,10 @@ def npairs_jackknife_xy_z(sample1, sample2, rp_bins, pi_bins, jtags1, jtags2, N_
     approx_cell1_size, approx_cell2_size = P.default_cell_sizes(
         approx_cell1_size, approx_cell2_size, (rp_max, rp_max, pi_max))
     xperiod, yperiod, zperiod = period
+    raw_same = _same_values(weights1, weights2) and _same_values(jtags1, jtags2)
     weights1, weights2, jtags1, jtags2 = _process_weights_jtags(sample1, sample2, weights1, weights2, jtags1,
                                                                 jtags2, N_samples)
     P.set_approximate_cell_sizes(approx_cell1_size, approx_cell2_size, period)
     P.check_search_lengths((rp_max, rp_max, pi_max), (xperiod, yperiod, zperiod))
     return _run(E.HP_NPAIRS_JACKKNIFE_XY_Z, sample1, sample2, (x1, y1, z1), (x2, y2, z2), period, pbc,
-                P.squared(rp_bins), P.squared(pi_bins), weights1, weights2, jtags1, jtags2, N_samples)
+                P.squared(rp_bins), P.squared(pi_bins), weights1, weights2, jtags1, jtags2, N_samples, raw_same)
