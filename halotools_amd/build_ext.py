@@ -50,7 +50,8 @@ def build(verbose=False, extra=None, out=None, jobs=None):
     jobs = jobs or min(len(SOURCES), max(1, min(os.cpu_count() or 1, 16)))
     with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
         objs = list(pool.map(compile_one, SOURCES))
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", out + ".tmp"]
+    # roctx: named phase ranges for rocprofv3 --marker-trace
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-lrocprofiler-sdk-roctx", "-o", out + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

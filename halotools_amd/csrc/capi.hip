@@ -15,6 +15,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "hp_internal.h"
 
 namespace hp {
@@ -491,7 +493,27 @@ static SortedSide lane_ptrs(char* ws, const Side& S, const double* x, const doub
     return o;
 }
 
+// Named host-side ranges of a request's phases (planning, cell build, pair
+// kernels, reductions) for timeline tools (rocprofv3 --marker-trace); a range
+// spans the enqueueing of that phase's work on the stream.  Without a tool
+// attached a push or pop costs a function call.
+struct PhaseRange {
+    bool open = false;
+    void begin(const char* name) {
+        end();
+        roctxRangePush(name);
+        open = true;
+    }
+    void end() {
+        if (open) roctxRangePop();
+        open = false;
+    }
+    ~PhaseRange() { end(); }
+};
+
 static int run(const hp_request* r, int kind) {
+    PhaseRange phase;
+    phase.begin("halopair: plan");
     Plan P;
     if (make_plan(r, kind, P)) return -1;
     if (!r->workspace) return fail("workspace must not be null");
@@ -538,6 +560,7 @@ static int run(const hp_request* r, int kind) {
     sb.vals[1] = reinterpret_cast<uint32_t*>(ws + P.o_vals[1]);
     sb.tile_counts = reinterpret_cast<uint32_t*>(ws + P.o_tiles);
     sb.digit_totals = reinterpret_cast<uint32_t*>(ws + P.o_digits);
+    phase.begin("halopair: cell build");
     if (r->build_events[0] && hipEventRecord(static_cast<hipEvent_t>(r->build_events[0]), s) != hipSuccess)
         return fail("event record failed");
     for (int sd = 0; sd < 2; sd++) {
@@ -634,6 +657,7 @@ static int run(const hp_request* r, int kind) {
     if (P.counts)
         launch_fill_u32(reinterpret_cast<uint32_t*>(L.pcount), 0u, (int64_t)P.M * P.crep * P.nb_count * 2, s);
     int rc = 0;
+    phase.begin("halopair: pair kernels");
     if (r->pair_events[0] && hipEventRecord(static_cast<hipEvent_t>(r->pair_events[0]), s) != hipSuccess)
         return fail("event record failed");
     if (!(P.marked && !P.sums && !P.counts && !P.rows1 && !P.elim)) {
@@ -668,6 +692,7 @@ static int run(const hp_request* r, int kind) {
     }
     if (r->pair_events[1] && hipEventRecord(static_cast<hipEvent_t>(r->pair_events[1]), s) != hipSuccess)
         return fail("event record failed");
+    phase.begin("halopair: reductions");
     uint64_t* red = reinterpret_cast<uint64_t*>(ws + P.o_red);
     if (P.counts) {
         if (launch_prefix_counts(L.pcount, P.Kr, kind == HP_NPAIRS_XY_Z ? P.Kp : 0, P.M, P.crep, r->counts, s))
