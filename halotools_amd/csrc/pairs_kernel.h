@@ -155,7 +155,7 @@ constexpr int kBatch = HP_BATCH;  // stream points per scalar-load wait
 #define HP_SPLIT_XY 1
 #endif
 #ifndef HP_COUNT_WAVES
-#define HP_COUNT_WAVES 6  // waves per SIMD the count kernels' registers are held to
+#define HP_COUNT_WAVES 7  // waves per SIMD the count kernels' registers are held to
 #endif
 #ifndef HP_PERSIST
 #define HP_PERSIST 1  // persistent workgroups (FP64-histogram modes) taking lane groups from work queues
