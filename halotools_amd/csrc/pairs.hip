@@ -109,7 +109,7 @@ int launch_pairs(const PairLaunch& L, hipStream_t s) {
                                 : (((L.mode == kModeWSum || L.mode == kModeWXYZ) && prod) ? 5
                                    : ((L.mode == kModeWSum || L.mode == kModeTan || L.mode == kModeWXYZ ||
                                        L.mode == kModeJack || L.mode == kModeJackXY) ? 4
-                                                                                     : (L.mode == kModeXYZ ? 5 : HP_COUNT_WAVES))));
+                                                                                     : (L.mode == kModeXYZ ? 5 : (L.mode == kModeObj ? 6 : HP_COUNT_WAVES)))));
     // shape for a launch of kc bins: sets A's layout, returns the waves per
     // workgroup (0 if nothing fits) and the resident waves per CU
     auto shape = [&](int kc, int* resident) -> int {

@@ -206,7 +206,7 @@ constexpr int kWavesPerSimd =
                           : ((MODE == kModeWSum || MODE == kModeTan || MODE == kModeWXYZ || MODE == kModeJack ||
                               MODE == kModeJackXY)
                                  ? 4
-                                 : (MODE == kModeXYZ ? 5 : HP_COUNT_WAVES)));
+                                 : (MODE == kModeXYZ ? 5 : (MODE == kModeObj ? 6 : HP_COUNT_WAVES))));
 
 // per-object counts: one u32 row set per lane (each lane owns its point's counts)
 
