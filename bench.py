@@ -624,6 +624,12 @@ def main():
             oracle.npairs_3d(pts, pts, rbins, period=L, num_threads=threads, fast=True)
             walls.append(time.perf_counter() - c0)
         wall = float(np.median(walls))
+        # the two phases of a call timed apart (median of 3): mesh build and
+        # argument handling (NumPy argsort), and the engine loop alone
+        phases = [oracle.npairs_3d_phases(pts, pts, rbins, period=L, num_threads=threads, fast=True)[1:]
+                  for _ in range(3)]
+        mesh_s = float(np.median([p[0] for p in phases]))
+        engine_s = float(np.median([p[1] for p in phases]))
         # single core: one call on the same input
         c0 = time.perf_counter()
         oracle.npairs_3d(pts, pts, rbins, period=L, num_threads=1, fast=True)
@@ -632,7 +638,9 @@ def main():
                "sample": "the full workload: reference mesh + engine loop restated in C (oracle/cpu_ref.c, "
                          "-Ofast, no -march), cell-1 range split over %d threads; median of 5 calls after a "
                          "warm-up, %.2f s each; counts checked equal to the GPU's" % (threads, wall),
-               "single_core_s": wall1, "single_core_value": ref_cand / wall1, "host": host}
+               "single_core_s": wall1, "single_core_value": ref_cand / wall1,
+               "mesh_build_s": mesh_s, "engine_only_s": engine_s, "engine_only_value": ref_cand / engine_s,
+               "host": host}
 
     legs = set(["batched"] + list(OTHER_LEGS)) if args.legs == "all" else set(args.legs.split(","))
     batched = None if (args.no_batched or "batched" not in legs) else bench_batched(dev, max(3, args.steps // 4))

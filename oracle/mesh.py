@@ -299,6 +299,27 @@ def npairs_3d(sample1, sample2, rbins, period=None, num_threads=1,
     return counts
 
 
+def npairs_3d_phases(sample1, sample2, rbins, period=None, num_threads=1, fast=False):
+    """npairs_3d with its two phases timed apart (for the CPU baseline):
+    returns (counts, seconds of the mesh build and argument handling,
+    seconds of the engine loop)."""
+    import time
+    t0 = time.perf_counter()
+    rbins = np.atleast_1d(rbins).astype("f8")
+    rmax = np.max(rbins)
+    mesh, cols = _prepare(sample1, sample2, (rmax, rmax, rmax), period, None, None, (rmax, rmax, rmax))
+    counts = np.zeros(len(rbins), dtype=np.int64)
+    mc = mesh.as_c()
+    ptrs = _ptrs(cols)
+    lib = load(fast)
+    t1 = time.perf_counter()
+    rc = lib.ref_npairs_3d(ctypes.byref(mc), *ptrs, rbins.ctypes.data, len(rbins), int(num_threads),
+                           counts.ctypes.data)
+    t2 = time.perf_counter()
+    assert rc == 0
+    return counts, t1 - t0, t2 - t1
+
+
 def npairs_per_object_3d(sample1, sample2, rbins, period=None, num_threads=1,
                          approx_cell1_size=None, approx_cell2_size=None, fast=False):
     """Oracle for PC/npairs_per_object_3d.py:20-137: (Npts1, K) counts per
