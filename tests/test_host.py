@@ -410,3 +410,20 @@ def test_noalloc_entry_rejects_before_device_work():
     assert lib.hp_npairs_3d_noalloc(ctypes.byref(req)) < 0
     assert "more than the library's fixed 67108864 bytes" in lib.hp_last_error().decode()
     assert lib.hp_npairs_3d_noalloc(None) < 0
+
+
+def test_noalloc_capacity_matches_documentation():
+    """The no-alloc entry's fixed scratch holds an auto-count of about 750K
+    points with 20 bins (the figure the header and the notes quote)."""
+    lib = E.library()
+
+    def size(n):
+        mock = E.HpMock(n1=n, n2=n, pbc=1)
+        mock.period[:] = [1000.0] * 3
+        r2 = np.ascontiguousarray(np.logspace(-1, np.log10(50), 20) ** 2)
+        req = E.HpRequest(kind=E.HP_NPAIRS_3D, nmocks=1, nr=20, flags=E.HP_SAME_SAMPLES)
+        req.mocks = ctypes.pointer(mock)
+        req.r2 = r2.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        return lib.hp_workspace_size(ctypes.byref(req))
+
+    assert 0 < size(720000) <= (64 << 20) < size(790000)
