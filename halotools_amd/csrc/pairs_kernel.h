@@ -213,6 +213,9 @@ constexpr int kWavesPerSimd =
 // Modes whose workgroups are persistent (measured on MI355X: the FP64-histogram
 // passes gain 2-7 %; the integer count passes lose 8-26 %, their register
 // budget leaving no room for the work loop's state without spilling).
+#ifndef HP_TAN_FMA
+#define HP_TAN_FMA 1  // forward-mode pass: t_i w_j + w_i t_j with one fused multiply-add
+#endif
 #ifndef HP_SPREAD
 #define HP_SPREAD 1  // spread lane order for the modes with wave-shared histogram rows
 #endif
@@ -778,7 +781,9 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                     } else if (MODE == kModeTan) {
                         // pair terms w_i w_j and t_i w_j + w_i t_j (t = dw/dtheta)
                         const double v = rl0 * bw[u];
-                        const double tv = rl1 * bw[u] + rl0 * bt[u];
+                        // one fused multiply-add (a tangent term, not a distance:
+                        // within the sums' 1e-12 tolerance, deterministic)
+                        const double tv = HP_TAN_FMA ? __builtin_fma(rl0, bt[u], rl1 * bw[u]) : rl1 * bw[u] + rl0 * bt[u];
                         __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a[u]), v, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
                         __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + A.tan_off + a[u]), tv, __ATOMIC_RELAXED,
@@ -994,7 +999,7 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                         const double h = (kb + u > lane) ? 1.0 : 0.5;
                         const double wj = bw[u] * h, tj = bt[u] * h;
                         const double v = lw[0] * wj;
-                        const double tv = lw[1] * wj + lw[0] * tj;
+                        const double tv = HP_TAN_FMA ? __builtin_fma(lw[0], tj, lw[1] * wj) : lw[1] * wj + lw[0] * tj;
                         __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a[u]), v, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
                         __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + A.tan_off + a[u]), tv, __ATOMIC_RELAXED,
