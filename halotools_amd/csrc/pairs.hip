@@ -56,6 +56,13 @@ static int wave_layout(const PairLaunch& L, int kc, int jslots, PairArgs& A) {
             A.cnt_off = (kc + 1) * 16 * 8;
             return A.cnt_off + (L.also_count ? (kc + 1) * 32 * 4 : 0);
         case kModeTan:
+            if (HP_TAN_INTERLEAVE) {
+                // per bin one block: W row, dW row (16 columns of 8 B each), then
+                // the count row (32 columns of 4 B) when counts are wanted
+                A.tan_off = 128;
+                A.cnt_off = 256;
+                return (kc + 1) * (L.also_count ? 384 : 256);
+            }
             A.tan_off = (kc + 1) * 16 * 8;
             A.cnt_off = 2 * A.tan_off;
             return A.cnt_off + (L.also_count ? (kc + 1) * 32 * 4 : 0);

@@ -213,6 +213,9 @@ constexpr int kWavesPerSimd =
 // Modes whose workgroups are persistent (measured on MI355X: the FP64-histogram
 // passes gain 2-7 %; the integer count passes lose 8-26 %, their register
 // budget leaving no room for the work loop's state without spilling).
+#ifndef HP_TAN_INTERLEAVE
+#define HP_TAN_INTERLEAVE 1  // forward-mode pass: a bin's W, dW and count rows in one block
+#endif
 #ifndef HP_TAN_FMA
 #define HP_TAN_FMA 1  // forward-mode pass: t_i w_j + w_i t_j with one fused multiply-add
 #endif
@@ -283,9 +286,18 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
     constexpr bool SHARE4 = (MODE == kModeWSum || MODE == kModeTan);
     constexpr bool SHARE2 = (MODE == kModeCount3D || MODE == kModeProj);
     const bool shared = (MODE == kModeXYZ) || (!F64 && A.shared_count);  // shared u32 counts
+    // The forward-mode pass keeps a bin's W row, dW/dtheta row and count row in
+    // one 384-byte block (128 B each), so the second and third atomics of a
+    // pair address the first one's block at fixed offsets (immediate offsets
+    // of the LDS instructions, no address arithmetic).
+    constexpr bool TANI = (MODE == kModeTan) && HP_TAN_INTERLEAVE;
+    constexpr uint32_t kTanRow = 128u, kTanCnt = 256u;
     const uint32_t step_r =
         XY ? (uint32_t)(Kp + 1) * cell_bytes
-           : (JK ? 8u : (shared ? 4u : ((SHARE4 || SHARE2) ? 128u : (MODE == kModeObj ? 256u : 512u))));
+           : (JK ? 8u
+                 : (shared ? 4u
+                           : (TANI ? (ALSO ? 384u : 256u)
+                                   : ((SHARE4 || SHARE2) ? 128u : (MODE == kModeObj ? 256u : 512u)))));
     const uint32_t lane_off =
         (shared || XY || JK) ? 0u
                : (SHARE4 ? (uint32_t)(lane & 15) * 8u
@@ -786,11 +798,12 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                         const double tv = HP_TAN_FMA ? __builtin_fma(rl0, bt[u], rl1 * bw[u]) : rl1 * bw[u] + rl0 * bt[u];
                         __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a[u]), v, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
-                        __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + A.tan_off + a[u]), tv, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + (TANI ? kTanRow : A.tan_off) + a[u]), tv,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         if (ALSO)
-                            __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(wsm + A.cnt_off + a[u] + cnt_col), rinc,
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_fetch_add(
+                                reinterpret_cast<uint32_t*>(wsm + (TANI ? kTanCnt : A.cnt_off) + a[u] + cnt_col), rinc,
+                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     } else {
                         __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a[u]), bw[u], __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1002,11 +1015,12 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                         const double tv = HP_TAN_FMA ? __builtin_fma(lw[0], tj, lw[1] * wj) : lw[1] * wj + lw[0] * tj;
                         __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a[u]), v, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
-                        __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + A.tan_off + a[u]), tv, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                        __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + (TANI ? kTanRow : A.tan_off) + a[u]), tv,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         if (ALSO) {
-                            __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(wsm + A.cnt_off + a[u] + cnt_col), inc,
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            __hip_atomic_fetch_add(
+                                reinterpret_cast<uint32_t*>(wsm + (TANI ? kTanCnt : A.cnt_off) + a[u] + cnt_col), inc,
+                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         }
                     } else if (MODE != kModeTan && (!XY || a[u] != trash_r)) {
                         accumulate(a[u], 0u, inc, false);
@@ -1024,7 +1038,9 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
             }
         } else if (MODE == kModeCount3D || MODE == kModeProj || ALSO) {
             for (int s = 0; s < kc; s++) {
-                const uint32_t c = (SHARE4 || SHARE2) ? (lane < 32 ? ch32[s * 32 + lane] : 0u) : ch32[s * 64 + lane];
+                const uint32_t c = TANI ? (lane < 32 ? ch32[s * 96 + lane] : 0u)  // 384-byte blocks
+                                        : ((SHARE4 || SHARE2) ? (lane < 32 ? ch32[s * 32 + lane] : 0u)
+                                                              : ch32[s * 64 + lane]);
                 const uint64_t t = wave_sum_u64((uint64_t)c);
                 if (lane == s % 64 && t) __hip_atomic_fetch_add(cacc + s, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
@@ -1108,11 +1124,12 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
         }
         if (MODE == kModeTan) {
             // rows of w_i w_j and of t_i w_j + w_i t_j, doubled under symmetry
-            const double* th = reinterpret_cast<const double*>(wsm + A.tan_off);
+            const double* th = reinterpret_cast<const double*>(wsm + (TANI ? kTanRow : A.tan_off));
             const double sc = sym ? 2.0 : 1.0;
             for (int s = 0; s < kc; s++) {
-                const double ws = wave_sum_f64(lane < 16 ? fh[s * 16 + lane] : 0.0);
-                const double ts = wave_sum_f64(lane < 16 ? th[s * 16 + lane] : 0.0);
+                const int rs = TANI ? (ALSO ? 48 : 32) : 16;  // doubles from one bin's W row to the next
+                const double ws = wave_sum_f64(lane < 16 ? fh[s * rs + lane] : 0.0);
+                const double ts = wave_sum_f64(lane < 16 ? th[s * rs + lane] : 0.0);
                 if (lane == 0) {
                     if (L.psum) L.psum[g * Kr + k0 + s] = sc * ws;
                     L.ptan[g * Kr + k0 + s] = sc * ts;
