@@ -271,6 +271,26 @@ def backward_rows(rows, gsuffix, offsets, out=None):
     return grad
 
 
+def zheng07_tangent(log_mass, mock_id, logMmin, sigma, out):
+    """out[:, 0] = Zheng07 central weights, out[:, 1] = their logMmin
+    derivatives, per point of mock mock_id[i] (one library kernel; the same
+    values as the elementwise torch expression)."""
+    n = log_mass.numel()
+    if out.shape != (n, 2) or not out.is_contiguous() or out.dtype != torch.float64:
+        raise ValueError("out must be a contiguous float64 (n, 2) tensor")
+    if log_mass.dtype != torch.float64 or mock_id.dtype != torch.int64 or logMmin.dtype != torch.float64:
+        raise ValueError("log_mass and logMmin must be float64, mock_id int64")
+    if not (log_mass.is_contiguous() and mock_id.is_contiguous() and logMmin.is_contiguous()):
+        raise ValueError("inputs must be contiguous")
+    if mock_id.numel() != n:
+        raise ValueError("mock_id must have one entry per point")
+    E.check(E.library().hp_zheng07_tangent(
+        ctypes.c_void_p(log_mass.data_ptr()), ctypes.c_void_p(mock_id.data_ptr()),
+        ctypes.c_void_p(logMmin.data_ptr()), float(sigma), n, ctypes.c_void_p(out.data_ptr()),
+        ctypes.c_void_p(torch.cuda.current_stream(out.device).cuda_stream)))
+    return out
+
+
 class SegmentSum(object):
     """Per-mock sums out[m] = values[offsets[m]:offsets[m+1]].sum() in a fixed
     order (library kernels, no atomics), with the scratch allocated once so the

@@ -34,7 +34,7 @@ EXPORTED_SYMBOLS = (
     "hp_npairs_projected", "hp_npairs_xy_z", "hp_marked_npairs_3d", "hp_npairs_per_object_3d",
     "hp_marked_npairs_xy_z", "hp_npairs_jackknife_3d", "hp_npairs_jackknife_xy_z", "hp_marked_backward",
     "hp_segment_sum_scratch", "hp_segment_sum", "hp_debug_bin_lookup", "hp_debug_hilbert",
-    "hp_npairs_3d_noalloc",
+    "hp_npairs_3d_noalloc", "hp_zheng07_tangent",
 )
 
 
@@ -121,6 +121,9 @@ def library():
     lib.hp_segment_sum.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32,
                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
     lib.hp_segment_sum.restype = ctypes.c_int
+    lib.hp_zheng07_tangent.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                       ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+    lib.hp_zheng07_tangent.restype = ctypes.c_int
     lib.hp_debug_bin_lookup.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int32,
                                         ctypes.POINTER(ctypes.c_double), ctypes.c_int64,
                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]

@@ -177,15 +177,16 @@ class HODBatchStep(object):
 
     def _body(self):
         b = self.b
-        u = (self.log_mass - self.logMmin[b.mock_id]) / self.sigma
-        w = 0.5 * (1.0 + torch.erf(u))
-        dw = -torch.exp(-u * u) / (math.sqrt(math.pi) * self.sigma)   # dw / dlogMmin
         if self.mode == "tangent":
-            self.wt[:, 0].copy_(w)
-            self.wt[:, 1].copy_(dw)
+            # weights and dw/dlogMmin in one library kernel (the same values as
+            # the torch expression of the reverse branch below)
+            D.zheng07_tangent(self.log_mass, b.mock_id, self.logMmin, self.sigma, self.wt)
             D.launch(self.res)
             torch.sum(self.res.tangents * self.c, dim=1, out=self.grad_logMmin)
         else:
+            u = (self.log_mass - self.logMmin[b.mock_id]) / self.sigma
+            w = 0.5 * (1.0 + torch.erf(u))
+            dw = -torch.exp(-u * u) / (math.sqrt(math.pi) * self.sigma)   # dw / dlogMmin
             self.w[:, 0].copy_(w)
             D.launch(self.res_counts)
             D.launch(self.res)

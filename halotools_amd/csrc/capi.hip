@@ -825,6 +825,17 @@ int hp_segment_sum(const double* values, const int64_t* offsets, int32_t nmocks,
     return 0;
 }
 
+int hp_zheng07_tangent(const double* log_mass, const int64_t* mock_id, const double* logMmin, double sigma,
+                       int64_t n, double* wt, void* stream) {
+    if (n < 0) return fail("n must be >= 0");
+    if (n > 0 && (!log_mass || !mock_id || !logMmin || !wt)) return fail("null argument");
+    if (!(sigma > 0.0) || !(sigma < INFINITY)) return fail("sigma must be positive and finite");
+    if (launch_zheng07_tangent(log_mass, mock_id, logMmin, sigma, n, wt, static_cast<hipStream_t>(stream)))
+        return fail("weights launch failed");
+    g_err.clear();
+    return 0;
+}
+
 // Host-only diagnostics: the bucketed bin lookup the kernels use, evaluated on
 // the CPU for a set of squared distances (tests pin it against searchsorted).
 int hp_debug_bin_lookup(const double* r2, int32_t K, const double* d, int64_t n, int32_t* out, int32_t* info) {

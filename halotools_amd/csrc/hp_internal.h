@@ -182,6 +182,8 @@ int launch_add_rows(const double* a, const double* b, double* out, int64_t n, hi
 int launch_pack_jack(const double* w, const int32_t* tags, int64_t n, int nsamp, double* out, int32_t* status,
                      hipStream_t s);
 int launch_jack_finish(double* acc, int nr, int np, int nmocks, int nsamp, double* out, hipStream_t s);
+int launch_zheng07_tangent(const double* logm, const int64_t* mock_id, const double* logMmin, double sigma,
+                           int64_t n, double* wt, hipStream_t s);
 
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ int64_t mock_of_point(const MockGeom* __restrict__ g, int nmocks, int side,

@@ -448,4 +448,31 @@ int launch_jack_finish(double* acc, int nr, int np, int nmocks, int nsamp, doubl
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Zheng07 central occupation of each point and its derivative with respect to
+// logMmin (empirical_models/occupation_models/zheng07_components.py:149-153):
+// u = (log_mass - logMmin[mock]) / sigma, w = 0.5 * (1 + erf(u)),
+// dw = -exp(-u * u) / (sqrt(pi) * sigma), written as (w, dw) rows: the weights
+// and tangents of the batched forward-mode step, in one pass over the points
+// (the divisions by the host constants are multiplications by their
+// reciprocals, as torch evaluates a tensor divided by a Python float).
+__global__ void k_zheng07_tangent(const double* __restrict__ logm, const int64_t* __restrict__ mock_id,
+                                  const double* __restrict__ logMmin, double inv_sigma, double inv_dnorm, int64_t n,
+                                  double2* __restrict__ wt) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double u = (logm[i] - logMmin[mock_id[i]]) * inv_sigma;
+    const double w = 0.5 * (1.0 + erf(u));
+    const double dw = -exp(-(u * u)) * inv_dnorm;
+    wt[i] = make_double2(w, dw);
+}
+
+int launch_zheng07_tangent(const double* logm, const int64_t* mock_id, const double* logMmin, double sigma,
+                           int64_t n, double* wt, hipStream_t s) {
+    if (n <= 0) return 0;
+    const double inv_sigma = 1.0 / sigma, inv_dnorm = 1.0 / (sqrt(M_PI) * sigma);
+    hipLaunchKernelGGL(k_zheng07_tangent, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, logm, mock_id, logMmin,
+                       inv_sigma, inv_dnorm, n, reinterpret_cast<double2*>(wt));
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 }  // namespace hp

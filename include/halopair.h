@@ -209,6 +209,16 @@ int hp_marked_backward(const double* rows, int32_t nr, const double* gsuffix,
  * scratch are device arrays; scratch holds hp_segment_sum_scratch(offsets,
  * nmocks) doubles (< 0 on error). */
 int64_t hp_segment_sum_scratch(const int64_t* offsets, int32_t nmocks);
+
+/* Zheng07 central weights and their logMmin tangents in one pass (no
+ * counterpart among the reference's pair counters: the batched model-fitting
+ * step's weight model, empirical_models/occupation_models/
+ * zheng07_components.py:149-153): for every point i of mock m = mock_id[i],
+ * u = (log_mass[i] - logMmin[m]) / sigma, wt[2i] = 0.5 * (1 + erf(u)),
+ * wt[2i+1] = -exp(-u*u) / (sqrt(pi) * sigma).  log_mass, mock_id, logMmin
+ * and wt are device arrays; the call only enqueues work on `stream`. */
+int hp_zheng07_tangent(const double* log_mass, const int64_t* mock_id, const double* logMmin, double sigma,
+                       int64_t n, double* wt, void* stream);
 int hp_segment_sum(const double* values, const int64_t* offsets, int32_t nmocks, double* out,
                    double* scratch, int64_t scratch_elems, void* stream);
 

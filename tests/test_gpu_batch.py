@@ -207,3 +207,25 @@ def test_tangent_pass_matches_reverse_mode_and_fd(gpu, same, period):
     np.testing.assert_allclose(res.tangents[0].cpu().numpy(), fd, rtol=1e-6, atol=1e-9)
     counts = npairs_3d(p1, p2, rb, period=period)
     assert torch.equal(res.counts[0], counts)
+
+
+def test_zheng07_tangent_kernel_matches_torch(gpu):
+    """The batched step's weights and logMmin tangents come from one library
+    kernel (hp_zheng07_tangent); they must equal the elementwise torch
+    expression of empirical_models/occupation_models/zheng07_components.py:149-153
+    bit for bit (same operations, same order, same erf/exp)."""
+    from halotools_amd import _device as D
+    rng = np.random.default_rng(11)
+    sizes = [7, 0, 5000, 1, 130000]
+    mock_id = torch.from_numpy(np.repeat(np.arange(len(sizes)), sizes)).to(gpu)
+    log_mass = torch.tensor(rng.uniform(10.0, 16.0, sum(sizes)), device=gpu)
+    logMmin = torch.tensor(rng.uniform(11.5, 12.5, len(sizes)), device=gpu)
+    for sigma in (0.3, 0.05, 1.7):
+        out = torch.empty((sum(sizes), 2), dtype=torch.float64, device=gpu)
+        D.zheng07_tangent(log_mass, mock_id, logMmin, sigma, out)
+        u = (log_mass - logMmin[mock_id]) / sigma
+        w = 0.5 * (1.0 + torch.erf(u))
+        dw = -torch.exp(-u * u) / (math.sqrt(math.pi) * sigma)
+        assert torch.equal(out[:, 0], w) and torch.equal(out[:, 1], dw), sigma
+    with pytest.raises(ValueError):
+        D.zheng07_tangent(log_mass, mock_id, logMmin, 0.3, torch.empty((3, 2), dtype=torch.float64, device=gpu))
