@@ -79,3 +79,39 @@ def test_batch_tile_invariant(gpu):
     for m, p in enumerate(pts):
         ph = p.cpu().numpy()
         assert np.array_equal(small[0][m].cpu().numpy(), ORC.npairs_3d(ph, ph, rb, period=L, num_threads=8))
+
+
+@pytest.mark.parametrize("tile", [2048, 8192])
+def test_batch_with_empty_mocks(gpu, tile):
+    """Mocks without points between others: their tile ranges are empty, and
+    every other mock's counts still equal the oracle's (multi-mock requests
+    through the C ABI, auto and cross)."""
+    from halotools_amd import _device as D, _engine as E
+    rng = np.random.default_rng(tile)
+    L = 100.0
+    rb = np.logspace(-1, np.log10(12.0), 9)
+    sizes1 = [0, 9000, 0, 700, 12000, 0]
+    sizes2 = [500, 8000, 0, 0, 3000, 0]
+    p1 = [rng.uniform(0, L, (n, 3)) for n in sizes1]
+    p2 = [rng.uniform(0, L, (n, 3)) for n in sizes2]
+
+    def cols(ps):
+        t = torch.tensor(np.concatenate(ps), device=gpu)
+        return tuple(t[:, i].contiguous() for i in range(3))
+
+    c1, c2 = cols(p1), cols(p2)
+
+    def run():
+        auto = D.run(E.HP_NPAIRS_3D, [dict(n1=a, n2=a, period=[L] * 3, pbc=True) for a in sizes1], c1, c1, rb * rb,
+                     same=True, device=gpu)
+        cross = D.run(E.HP_NPAIRS_3D, [dict(n1=a, n2=b, period=[L] * 3, pbc=True) for a, b in zip(sizes1, sizes2)],
+                      c1, c2, rb * rb, device=gpu)
+        return auto.counts.cpu().numpy(), cross.counts.cpu().numpy()
+
+    auto, cross = _with_tile(tile, run)
+    for m in range(len(sizes1)):
+        want = ORC.npairs_3d(p1[m], p1[m], rb, period=L, num_threads=8) if sizes1[m] else np.zeros(len(rb), np.int64)
+        assert np.array_equal(auto[m], want), m
+        both = sizes1[m] and sizes2[m]
+        want = ORC.npairs_3d(p1[m], p2[m], rb, period=L, num_threads=8) if both else np.zeros(len(rb), np.int64)
+        assert np.array_equal(cross[m], want), m
