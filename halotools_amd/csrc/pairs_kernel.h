@@ -594,8 +594,14 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                 }
                 return;
             }
-            __hip_atomic_fetch_add(reinterpret_cast<double*>(wsm + a), v, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            // all-pairs row: in 3-D its 16 lane columns (no two lanes of one LDS
+            // pass on one address), summed at the flush; (rp, pi) keeps one row
+            if (!XY)
+                __hip_atomic_fetch_add(reinterpret_cast<double*>(wsm + A.jr0_off + a * 16u + r0col), v,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            else
+                __hip_atomic_fetch_add(reinterpret_cast<double*>(wsm + a), v, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
             const bool same_t = (ltag == ts);
             const double lv = same_t ? v : 0.5 * v;
             if (lslot > 0)
@@ -1106,7 +1112,13 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                 const double* row = reinterpret_cast<const double*>(wsm + r * A.jrow);
                 double* dst = L.jacc + ((int64_t)m * (L.nsamp + 1) + tag) * nbj;
                 for (int c = lane; c < cells; c += 64) {
-                    const double val = row[c];
+                    double val = row[c];
+                    if (!XY && r == 0) {
+                        // 3-D all-pairs row: the sum of its 16 lane columns
+                        const double* r0 = reinterpret_cast<const double*>(wsm + A.jr0_off + c * 128);
+                        val = 0.0;
+                        for (int k = 0; k < 16; k++) val += r0[k];
+                    }
                     if (val == 0.0) continue;
                     int64_t b = c;
                     if (XY) {
