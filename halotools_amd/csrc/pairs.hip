@@ -35,7 +35,7 @@ static int wave_layout(const PairLaunch& L, int kc, int jslots, PairArgs& A) {
             if (L.br.nbins > 64) { A.shared_count = 1; return (kc + 1) * 4; }
             return (kc + 1) * 32 * 4;  // rows shared by lanes l and l+32
         case kModeXYZ: return (kc + 1) * (Kp + 1) * 4;
-        case kModeWXYZ: return (kc + 1) * (Kp + 1) * 8;
+        case kModeWXYZ: return HP_WXYZ_COMPACT ? kc * Kp * 8 : (kc + 1) * (Kp + 1) * 8;
         case kModeJack:
         case kModeJackXY: {
             // row 0 plus one row per subsample tag the wave keeps in LDS

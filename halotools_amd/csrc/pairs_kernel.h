@@ -213,6 +213,9 @@ constexpr int kWavesPerSimd =
 // Modes whose workgroups are persistent (measured on MI355X: the FP64-histogram
 // passes gain 2-7 %; the integer count passes lose 8-26 %, their register
 // budget leaving no room for the work loop's state without spilling).
+#ifndef HP_WXYZ_COMPACT
+#define HP_WXYZ_COMPACT 1  // marked (rp, pi) table without trash row and column
+#endif
 #ifndef HP_TAN_INTERLEAVE
 #define HP_TAN_INTERLEAVE 1  // forward-mode pass: a bin's W, dW and count rows in one block
 #endif
@@ -292,8 +295,12 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
     // of the LDS instructions, no address arithmetic).
     constexpr bool TANI = (MODE == kModeTan) && HP_TAN_INTERLEAVE;
     constexpr uint32_t kTanRow = 128u, kTanCnt = 256u;
+    // The marked (rp, pi) pass never writes its trash row or column (pairs
+    // outside (rp_max, pi_max) skip the atomics), so its table has Kr x Kp
+    // cells only: 20 waves per CU fit in the LDS instead of 16.
+    constexpr bool WXC = (MODE == kModeWXYZ) && HP_WXYZ_COMPACT;
     const uint32_t step_r =
-        XY ? (uint32_t)(Kp + 1) * cell_bytes
+        XY ? (uint32_t)(WXC ? Kp : Kp + 1) * cell_bytes
            : (JK ? 8u
                  : (shared ? 4u
                            : (TANI ? (ALSO ? 384u : 256u)
@@ -1056,7 +1063,7 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
             // the group's (rp, pi) cells without the trash row and column
             const int nc = kc * Kp;
             double* dst = L.psum + g * (int64_t)Kr * Kp + (int64_t)k0 * Kp;
-            for (int s = lane; s < nc; s += 64) dst[s] = fh[(s / Kp) * (Kp + 1) + s % Kp];
+            for (int s = lane; s < nc; s += 64) dst[s] = WXC ? fh[s] : fh[(s / Kp) * (Kp + 1) + s % Kp];
         }
         if (MODE == kModeWSum) {
             if (L.psum) {
