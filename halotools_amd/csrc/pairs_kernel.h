@@ -228,6 +228,9 @@ constexpr int kWavesPerSimd =
 #ifndef HP_QUEUES
 #define HP_QUEUES 8  // work queues of a persistent launch (one per XCD)
 #endif
+#ifndef HP_XCD_CHUNK
+#define HP_XCD_CHUNK 64  // non-persistent modes: blocks per XCD run (0: blocks in dispatch order)
+#endif
 #ifndef HP_PERSIST_COUNT
 #define HP_PERSIST_COUNT 0
 #endif
@@ -347,8 +350,11 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
     // sees one region; a wave whose queue is empty moves on to the next
     // XCD's.  Every wave leaves once all eight are exhausted (one shared
     // queue measured 4-13 % slower).  Other modes launch one wave per lane
-    // group, in group order (a remap giving each XCD a contiguous run of
-    // groups measured equal on one box and 8 % slower on the 64-mock batch).
+    // group; XCD x takes runs of HP_XCD_CHUNK consecutive blocks (see below),
+    // which cut the L2 fetches of the 1M-point count from 1.8 GB to 0.65 GB
+    // and the 64-mock batch count by 3.5 % (one contiguous eighth per XCD
+    // fetched less still but ran 8 % slower on that batch: its XCDs finished
+    // unevenly).
     int qn = 0;
     const int xq = (int)(blockIdx.x & 7u);
     for (;;) {
@@ -372,7 +378,21 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
             for (int i = lane; i < A.wave_bytes / 4; i += 64) h[i] = 0u;
             __builtin_amdgcn_wave_barrier();
         } else {
-            g = (int64_t)blockIdx.x * WAVES + wid;
+            int64_t b = blockIdx.x;
+#if HP_XCD_CHUNK > 0
+            {
+                // chunked XCD interleave: XCD x (= b % 8) runs its j-th block
+                // (j = b / 8) on block (j / C) * 8C + x * C + j % C, so each
+                // XCD works through runs of C consecutive blocks (a compact
+                // region of the Hilbert-ordered groups) while every XCD still
+                // gets runs spread over the whole range; blocks past the last
+                // full 8C super-run keep their own index (a bijection)
+                constexpr int64_t C = HP_XCD_CHUNK;
+                const int64_t full = ((int64_t)gridDim.x / (8 * C)) * (8 * C);
+                if (b < full) b = ((b >> 3) / C) * (8 * C) + (b & 7) * C + ((b >> 3) % C);
+            }
+#endif
+            g = b * WAVES + wid;
             if (g >= L.ngroups) break;
         }
         const double* gedges_r = reinterpret_cast<const double*>(static_cast<const char*>(L.ws) + L.br.edges_off);
