@@ -186,7 +186,7 @@ def bench_batched(dev, steps):
             "modes_check": {"batched_reverse_counts_equal": rev_equal, "batched_reverse_grad_rel": rev_grad_rel,
                             "single_tangent_counts_equal": tan_ok, "single_tangent_grad_rel": tan_rel,
                             "single_reverse_counts_equal": rev_ok, "single_reverse_grad_rel": rev_rel},
-            "autotune": {"seconds": t_tune, "tile_waves": b.tile_waves,
+            "autotune": {"seconds": t_tune, "tile_waves": b.waves_for(autotune.TANGENT),
                          "buckets": {k: {"target_points_per_cell": v[0][0], "waves": v[0][1]}
                                      for k, v in tuned.items()}},
             "cell_build": _build_rate(npts, build_s)}

@@ -4,17 +4,19 @@ The reference counts one sample at a time: every call builds its own mesh and
 runs its engine over it (PC/npairs_3d.py:111-150; PC =
 halotools/mock_observables/pair_counters), and model fitting calls it once per
 mock (halotools/empirical_models/factories/mock_factory_template.py:288-320).
-Here all mocks are concatenated: one counting sort builds every mock's cells
-(keys carry a per-mock offset), one pair-kernel launch covers the lane groups
-of all mocks (each group knows its mock's geometry), and the per-mock outputs
-come out of one fixed-order reduction.
+Here all mocks are concatenated: one segmented counting sort builds every
+mock's cells (keys local to a mock, sort tiles that never cross one), one
+pair-kernel launch covers the lane groups of all mocks (each group knows its
+mock's geometry), and the per-mock outputs come out of one fixed-order
+reduction.
 
 ``HODBatchStep`` is the model-fitting step of the benchmark: for every mock,
 the npairs_3d auto-counts, the product-weighted marked sums with Zheng07
 central-occupation weights, and the gradient of a loss sum_k c_k W_k with
-respect to each mock's logMmin -- a count pass, a marked pass producing the
-Jacobian rows, the backward contraction and a fixed-order per-mock sum -- all
-capturable in one HIP graph.
+respect to each mock's logMmin, all capturable in one HIP graph: by default
+one forward-mode pair pass carrying (w, dw/dlogMmin) per point; or, in
+reverse mode, a count pass, a marked pass producing the Jacobian rows, the
+backward contraction and a fixed-order per-mock sum.
 """
 import math
 
@@ -33,6 +35,9 @@ class MockBatch(object):
     points : list of (N_m, 3) arrays/tensors, or one (sum N_m, 3) array with `sizes`
     period : box length (scalar or 3-vector), the same for every mock
     rbins  : radial bin edges (validated like npairs_3d's rbins)
+    cells  : optional cell edge per mock (else the tuner's cache, else the library default)
+    tile_waves : optional wavefronts per pair-kernel workgroup for every launch
+             (else the tuned shape of each launch's kind, else the library's choice)
     """
 
     def __init__(self, points, period, rbins, sizes=None, device=None, cells=None, tile_waves=0):
@@ -56,26 +61,39 @@ class MockBatch(object):
         rmax = float(np.max(self.rbins))
         P.check_search_lengths((rmax, rmax, rmax), tuple(self.period))
         self.rmax = rmax
+        # the caller's workgroup shape for every launch (0: per kind, see waves_for)
         self.tile_waves = int(tile_waves)
+        self._kind_waves = {}  # tuned workgroup shape of the heaviest mock, per tuner kind
         self.mocks = [dict(n1=n, n2=n, period=list(self.period), pbc=True, cell=[0.0] * 3) for n in self.sizes]
         if cells is not None:
             for m in range(self.M):
                 self.mocks[m]["cell"] = [float(cells[m])] * 3
         else:
-            self._apply_tuning(E.HP_NPAIRS_3D, keep_waves=bool(tile_waves))
+            self._apply_tuning(E.HP_NPAIRS_3D)
         mock_id = np.repeat(np.arange(self.M), self.sizes)
         self.mock_id = torch.from_numpy(mock_id).to(self.device)
 
-    def _apply_tuning(self, kind, keep_waves=False):
+    def _apply_tuning(self, kind):
         # per-mock cell edges from the tuner's cache; the workgroup shape (one
-        # per launch) from the entry of the mock with the most pair work
+        # per launch) from the entry of the mock with the most pair work, kept
+        # for launches of the tuned kind only (the shape that suits one kernel
+        # mode can starve another: 8-wave workgroups of the per-lane Jacobian
+        # rows leave one workgroup per CU)
         heaviest = int(np.argmax(self.sizes)) if self.M else 0
         for m, n in enumerate(self.sizes):
             hit = autotune.lookup(kind, n, self.period, self.rmax)
             if hit is not None:
                 self.mocks[m]["cell"] = [hit[0]] * 3
-                if m == heaviest and not keep_waves:
-                    self.tile_waves = hit[1]
+                if m == heaviest:
+                    self._kind_waves[kind] = hit[1]
+
+    def waves_for(self, kind):
+        """Workgroup waves for launches of tuner kind `kind` (E.HP_NPAIRS_3D,
+        E.HP_MARKED_NPAIRS_3D or autotune.TANGENT): the caller's tile_waves when
+        given, else the tuned shape of that kind, else 0 (the library's choice)."""
+        if self.tile_waves:
+            return self.tile_waves
+        return self._kind_waves.get(kind, 0)
 
     def tune(self, kind=autotune.TANGENT, **kw):
         """Autotune cell edges and the tile shape on this batch's own mocks.
@@ -104,7 +122,7 @@ class MockBatch(object):
         recorded around the cell build and the pair kernel (profiling).
         """
         res = D.run(E.HP_NPAIRS_3D, self.mocks, self.cols, self.cols, self.rbins * self.rbins, same=True,
-                    device=self.device, tile_waves=self.tile_waves, build_events=build_events,
+                    device=self.device, tile_waves=self.waves_for(E.HP_NPAIRS_3D), build_events=build_events,
                     pair_events=pair_events)
         self._check(res)
         return res.counts
@@ -114,7 +132,7 @@ class MockBatch(object):
         w = weights.to(self.device, torch.float64)
         plan = dict(mocks=self.mocks, s1=self.cols, s2=self.cols, r2=self.rbins * self.rbins, device=self.device,
                     same_weights=True, offsets1=list(self.offsets), offsets2=list(self.offsets),
-                    tile_waves=self.tile_waves)
+                    tile_waves=self.waves_for(E.HP_MARKED_NPAIRS_3D))
         return _MarkedProductSums.apply(w, w, plan)
 
     @staticmethod
@@ -157,7 +175,7 @@ class HODBatchStep(object):
             self.wt = torch.empty((n, 2), dtype=torch.float64, device=dev)
             self.res = D.prepare(E.HP_MARKED_NPAIRS_3D, batch.mocks, batch.cols, batch.cols, r2, same=True,
                                  w1=self.wt, nw=2, wfunc=1, want_counts=False, want_sums=True, also_counts=True,
-                                 tangent=True, device=dev, tile_waves=batch.tile_waves)
+                                 tangent=True, device=dev, tile_waves=batch.waves_for(autotune.TANGENT))
             self.res_counts = self.res
         else:
             self.w = torch.empty((n, 1), dtype=torch.float64, device=dev)
@@ -167,10 +185,10 @@ class HODBatchStep(object):
             # counts from the (symmetric) count pass; the marked pass keeps only
             # its FP64 histograms in LDS, which leaves room for more waves per CU
             self.res_counts = D.prepare(E.HP_NPAIRS_3D, batch.mocks, batch.cols, batch.cols, r2, same=True,
-                                        device=dev)
+                                        device=dev, tile_waves=batch.waves_for(E.HP_NPAIRS_3D))
             self.res = D.prepare(E.HP_MARKED_NPAIRS_3D, batch.mocks, batch.cols, batch.cols, r2, same=True,
                                  w1=self.w, nw=1, wfunc=1, want_counts=False, want_sums=True, rows1=True,
-                                 rows2=True, device=dev, tile_waves=batch.tile_waves)
+                                 rows2=True, device=dev, tile_waves=batch.waves_for(E.HP_MARKED_NPAIRS_3D))
             self.dtheta = torch.empty(n, dtype=torch.float64, device=dev)
             self.segsum = D.SegmentSum(batch.offsets, dev)
         self.graph = None

@@ -110,7 +110,10 @@ def test_autotune_keeps_results(data, gpu, tmp_path, monkeypatch, which):
     kind = E.HP_MARKED_NPAIRS_3D if which == "reverse" else autotune.TANGENT
     tuned = b.tune(kind, reps=1, targets=(2.0, 8.0), waves=(2, 8))
     assert tuned and (tmp_path / "tune.json").exists()
-    assert b.tile_waves in (2, 8)
+    assert b.waves_for(kind) in (2, 8)
+    # the shape tuned for one kind is not applied to launches of another
+    other = autotune.TANGENT if which == "reverse" else E.HP_MARKED_NPAIRS_3D
+    assert b.waves_for(other) == 0 and b.waves_for(E.HP_NPAIRS_3D) == 0
     assert torch.equal(b.npairs_3d(), before)
     step = HODBatchStep(b, torch.cat(lm), logMmin, 0.3, c)
     counts = step.run()[0]
