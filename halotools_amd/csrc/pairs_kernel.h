@@ -422,16 +422,17 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
         const int pbc = G.pbc;
         const int64_t cell_off = G.cell_off;
 
-        // ---- bounding box of the group (f32, rounded outwards)
-        const float INF = __builtin_inff();
-        const float bxlo = unif(wave_min(act ? f32_down(px) : INF));
-        const float bylo = unif(wave_min(act ? f32_down(py) : INF));
-        const float bzlo = unif(wave_min(act ? f32_down(pz) : INF));
-        const float bxhi = unif(wave_max(act ? f32_up(px) : -INF));
-        const float byhi = unif(wave_max(act ? f32_up(py) : -INF));
-        const float bzhi = unif(wave_max(act ? f32_up(pz) : -INF));
-
+        // ---- bounding box of the group (f32, rounded outwards), grown by the
+        // eps margin (1e-5 of the box size; the f32 sums of the window and
+        // filter tests below round by ~1e-7 of it)
         const float eps = G.eps;
+        const float INF = __builtin_inff();
+        const float bxlo = unif(wave_min(act ? f32_down(px) : INF) - eps);
+        const float bylo = unif(wave_min(act ? f32_down(py) : INF) - eps);
+        const float bzlo = unif(wave_min(act ? f32_down(pz) : INF) - eps);
+        const float bxhi = unif(wave_max(act ? f32_up(px) : -INF) + eps);
+        const float byhi = unif(wave_max(act ? f32_up(py) : -INF) + eps);
+        const float bzhi = unif(wave_max(act ? f32_up(pz) : -INF) + eps);
         const float Rx = G.search[0], Ry = G.search[1], Rz = G.search[2];
         const float cxf = (float)G.cell[0], cyf = (float)G.cell[1], czf = (float)G.cell[2];
         const float r2xy = G.r2xy;
@@ -890,11 +891,11 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
         };
 
         for (int jx = jx0; jx <= jx1; jx++) {
-            const float gx0 = fmaxf(0.f, fmaxf((float)jx * cxf - bxhi, bxlo - (float)(jx + 1) * cxf) - eps);
+            const float gx0 = fmaxf(0.f, fmaxf((float)jx * cxf - bxhi, bxlo - (float)(jx + 1) * cxf));
             const int tx = jx < 0 ? -1 : (jx >= nx ? 1 : 0);
             const int wx = jx - tx * nx;
             for (int jy = jy0; jy <= jy1; jy++) {
-                const float gy0 = fmaxf(0.f, fmaxf((float)jy * cyf - byhi, bylo - (float)(jy + 1) * cyf) - eps);
+                const float gy0 = fmaxf(0.f, fmaxf((float)jy * cyf - byhi, bylo - (float)(jy + 1) * cyf));
                 const float d2 = gx0 * gx0 + gy0 * gy0;
                 if (d2 > r2xy) continue;
                 const float dzr = (shape == 0) ? sqrtf(fmaxf(r2xy - d2, 0.f)) + eps : Rz;
@@ -938,8 +939,8 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                     // search length; survivors go to the ring and are evaluated
                     // against all 64 lane points in batches of B.  The image of
                     // stream point q visited here sits at q + t*L for either
-                    // orientation.  f32 with the box rounded outwards and an eps
-                    // margin keeps every in-range pair.
+                    // orientation.  f32 with the box rounded outwards and grown by
+                    // the eps margin keeps every in-range pair.
                     const int img = (sxs + 1) * 9 + (sys + 1) * 3 + (szs + 1);
                     if (img != rimg) {
                         drain();
@@ -966,9 +967,9 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                             const float fx = (float)SX[jj] + ftx;
                             const float fy = (float)SY[jj] + fty;
                             const float fz = (float)SZ[jj] + ftz;
-                            const float gx = fmaxf(0.f, fmaxf(fx - bxhi, bxlo - fx) - eps);
-                            const float gy = fmaxf(0.f, fmaxf(fy - byhi, bylo - fy) - eps);
-                            const float gz = fmaxf(0.f, fmaxf(fz - bzhi, bzlo - fz) - eps);
+                            const float gx = fmaxf(0.f, fmaxf(fx - bxhi, bxlo - fx));
+                            const float gy = fmaxf(0.f, fmaxf(fy - byhi, bylo - fy));
+                            const float gz = fmaxf(0.f, fmaxf(fz - bzhi, bzlo - fz));
                             const float gxy = gx * gx + gy * gy;
                             ok = (shape == 0) ? (gxy + gz * gz <= r2xy) : (gxy <= r2xy && gz <= Rz);
                             if (after_group) ok = ok && HR[jj] >= hend;
