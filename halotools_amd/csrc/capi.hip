@@ -151,7 +151,7 @@ struct Plan {
     size_t o_pcount = 0, o_psum = 0, o_pevals = 0, o_red = 0, o_status = 0;
     bool elim = false;
     size_t o_corr = 0, o_rows_tmp = 0;
-    size_t o_hrank = 0;              // symmetric auto-counts: Hilbert ranks of the row-major order
+    size_t o_hkey = 0;               // symmetric auto-counts: Hilbert order keys of the row-major order
     size_t o_rows[2] = {0, 0};       // packed coordinate rows of each side with a row-major order
     int row_stride[2] = {4, 4}, row_nw[2] = {0, 0};
     size_t o_keys_row = 0;           // row-major keys of a side sorted in both orders
@@ -436,7 +436,7 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     // unweighted auto-counts evaluate each unshifted unordered pair once; they
     // need every row-major point's position in the Hilbert order
     P.sym = P.same && !P.obj && (!P.marked || P.tangent);
-    if (P.sym && P.n[0] > 0) P.o_hrank = take((size_t)P.n[0] * 4);
+    if (P.sym && P.n[0] > 0) P.o_hkey = take((size_t)P.n[0] * 8);
     // packed rows (x, y, z, Hilbert position[, up to 4 weights]) of every side
     // the stream side is built from: one random row read per point in the gather
     for (int s = 0; s < 2; s++) {
@@ -568,13 +568,14 @@ static int run(const hp_request* r, int kind) {
             const Side& S = P.sorted[sd][o];
             if (!S.needed) continue;
             // with SAME_SAMPLES only side 0 exists; the geometry rows of side 0 describe it.
-            // Symmetric auto-counts: the Hilbert build (first) leaves each point's
-            // Hilbert position, the row-major gather turns it into the ranks.
+            // Symmetric auto-counts: the Hilbert key pass (first) leaves each
+            // point's Hilbert key in its row, the row-major gather turns it into
+            // the order keys.
             const bool rank = P.sym && sd == 0 && P.n[0] > 0;
             // a side sorted in both orders computes its keys once (Hilbert build first)
             const bool both = P.sorted[sd][kHilbert].needed && P.sorted[sd][kRowMajor].needed;
             uint32_t* krow = both ? reinterpret_cast<uint32_t*>(ws + P.o_keys_row) : nullptr;
-            uint32_t* hrank = rank ? reinterpret_cast<uint32_t*>(ws + P.o_hrank) : nullptr;
+            uint64_t* hkey = rank ? reinterpret_cast<uint64_t*>(ws + P.o_hkey) : nullptr;
             PackedRows rows;
             if (P.sorted[sd][kRowMajor].needed) {
                 rows.p = reinterpret_cast<double*>(ws + P.o_rows[sd]);
@@ -588,13 +589,13 @@ static int run(const hp_request* r, int kind) {
             st.ntiles = P.ntiles[sd];
             if (build_sorted_side(dgeom, P.M, sd, (Order)o, X[sd], Y[sd], Z[sd], P.weighted() ? W[sd] : nullptr, P.nw,
                                   P.n[sd], P.total_cells, o == kRowMajor ? P.bits_row : P.bits_hilbert, sb, st,
-                                  side_ptrs(ws, S), status, s, rows, rank && o == kHilbert,
-                                  o == kRowMajor ? hrank : nullptr, o == kHilbert ? krow : nullptr,
+                                  side_ptrs(ws, S), status, s, rows,
+                                  o == kRowMajor ? hkey : nullptr, o == kHilbert ? krow : nullptr,
                                   o == kRowMajor ? krow : nullptr))
                 return fail("cell build launch failed");
         }
     }
-    const uint32_t* hrank = (P.sym && P.n[0] > 0) ? reinterpret_cast<const uint32_t*>(ws + P.o_hrank) : nullptr;
+    const uint64_t* hkey = (P.sym && P.n[0] > 0) ? reinterpret_cast<const uint64_t*>(ws + P.o_hkey) : nullptr;
     if (r->build_events[1] && hipEventRecord(static_cast<hipEvent_t>(r->build_events[1]), s) != hipSuccess)
         return fail("event record failed");
 
@@ -615,7 +616,9 @@ static int run(const hp_request* r, int kind) {
     L.ws = ws;
     L.wfunc = r->wfunc;
     L.waves = r->tile_waves;
-    L.hrank = hrank;
+    L.hkey = hkey;
+    L.hrows = hkey ? reinterpret_cast<const double*>(ws + P.o_rows[0]) : nullptr;
+    L.hrow_stride = P.row_stride[0];
     L.pcount = P.counts ? reinterpret_cast<uint64_t*>(ws + P.o_pcount) : nullptr;
     L.psum = P.sums ? reinterpret_cast<double*>(ws + P.o_psum) : nullptr;
     L.ptan = P.tangent ? reinterpret_cast<double*>(ws + P.o_ptan) : nullptr;
@@ -686,7 +689,8 @@ static int run(const hp_request* r, int kind) {
         B.pcount = nullptr;
         B.psum = nullptr;
         B.pevals = nullptr;
-        B.hrank = nullptr;
+        B.hkey = nullptr;
+        B.hrows = nullptr;
         // with SAME_SAMPLES the side-1 rows of the geometry equal the side-0 rows
         if (launch_pairs(B, s)) return fail("pair kernel (swapped) launch failed");
     }

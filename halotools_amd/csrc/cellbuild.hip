@@ -115,18 +115,6 @@ __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, 
     const int64_t m = (mf == ml) ? mf : mock_of_point(geom, nmocks, side, p);
     const MockGeom& G = geom[m];
     const double px = x[p], py = y[p], pz = z[p];
-    if (rows.p) {
-        double* r = rows.p + p * rows.stride;
-        reinterpret_cast<double4*>(r)[0] = make_double4(px, py, pz, 0.0);
-        if (rows.nw > 0) {
-            double4 wr = make_double4(0.0, 0.0, 0.0, 0.0);
-            wr.x = w[p * nw];
-            if (rows.nw > 1) wr.y = w[p * nw + 1];
-            if (rows.nw > 2) wr.z = w[p * nw + 2];
-            if (rows.nw > 3) wr.w = w[p * nw + 3];
-            reinterpret_cast<double4*>(r)[1] = wr;
-        }
-    }
     bool bad = false;
     const int ix = cell_index(px, G.inv_cell[0], G.ndiv[0], bad, G.period[0], G.pbc);
     const int iy = cell_index(py, G.inv_cell[1], G.ndiv[1], bad, G.period[1], G.pbc);
@@ -149,6 +137,21 @@ __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, 
         key = hilbert3((uint32_t)hx, (uint32_t)hy, (uint32_t)hz, hb);
     }
     keys[p] = key;
+    if (rows.p) {
+        // the row's fourth word: the Hilbert key (Hilbert order), which with the
+        // point's index fixes its place in the stable Hilbert sort
+        double* r = rows.p + p * rows.stride;
+        const double slot = __hiloint2double(0, order == kRowMajor ? 0 : (int)key);
+        reinterpret_cast<double4*>(r)[0] = make_double4(px, py, pz, slot);
+        if (rows.nw > 0) {
+            double4 wr = make_double4(0.0, 0.0, 0.0, 0.0);
+            wr.x = w[p * nw];
+            if (rows.nw > 1) wr.y = w[p * nw + 1];
+            if (rows.nw > 2) wr.z = w[p * nw + 2];
+            if (rows.nw > 3) wr.w = w[p * nw + 3];
+            reinterpret_cast<double4*>(r)[1] = wr;
+        }
+    }
     if (bad && status) atomicOr(status, 1);
 }
 
@@ -274,8 +277,7 @@ __global__ void __launch_bounds__(WAVES * 64) k_radix_scatter(const uint32_t* __
                                                               const uint32_t* __restrict__ tile_first, int nmocks,
                                                               int side, int shift, int add_cell_off,
                                                               const uint32_t* __restrict__ tile_counts,
-                                                              const uint32_t* __restrict__ digit_totals, int ntiles,
-                                                              double* __restrict__ inv_rows, int inv_stride) {
+                                                              const uint32_t* __restrict__ digit_totals, int ntiles) {
     constexpr int NT = WAVES * 64;
     constexpr int kPerWave = TILE / WAVES;
     constexpr int kRounds = kPerWave / 64;
@@ -376,23 +378,23 @@ __global__ void __launch_bounds__(WAVES * 64) k_radix_scatter(const uint32_t* __
         const uint32_t pos = gbase[dg] + (uint32_t)i - toff[dg];
         kout[pos] = key + koff;
         vout[pos] = sval[i];
-        // last pass: the sorted position of each input point, into its packed row
-        if (inv_rows) reinterpret_cast<uint32_t*>(inv_rows + (int64_t)sval[i] * inv_stride)[6] = pos;
     }
 }
 
 // Sorted copies of coordinates (and weight rows) in the new order: one packed
-// row per point (x, y, z, the Hilbert position, and the weights when they
-// fit in the row) read at its input position.
+// row per point (x, y, z, the Hilbert key, and the weights when they fit in
+// the row) read at its input position.  hkey: the Hilbert order key of each
+// point, (Hilbert key << 32) | input index -- the stable Hilbert sort orders
+// the points of a mock by exactly this value.
 __global__ void k_gather(const uint32_t* __restrict__ perm, int64_t n, PackedRows rows, const double* __restrict__ w,
                          int nw, double* __restrict__ ox, double* __restrict__ oy, double* __restrict__ oz,
-                         double* __restrict__ ow, uint32_t* __restrict__ hrank) {
+                         double* __restrict__ ow, uint64_t* __restrict__ hkey) {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q >= n) return;
     const int64_t p = perm[q];
     const double* r = rows.p + p * rows.stride;
     const double4 c = reinterpret_cast<const double4*>(r)[0];
-    if (hrank) hrank[q] = (uint32_t)__double2loint(c.w);
+    if (hkey) hkey[q] = ((uint64_t)(uint32_t)__double2loint(c.w) << 32) | (uint64_t)(uint32_t)p;
     ox[q] = c.x;
     oy[q] = c.y;
     oz[q] = c.z;
@@ -435,7 +437,7 @@ int sort_tile(int64_t n) {
 int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, const double* x, const double* y,
                       const double* z, const double* w, int nw, int64_t n, int64_t total_cells, int key_bits,
                       SortBufs sb, SortTiles st, SortedSide out, int32_t* status, hipStream_t s, PackedRows rows,
-                      bool hpos_to_rows, uint32_t* hrank_out, uint32_t* keys_row_out, const uint32_t* keys_in) {
+                      uint64_t* hkey_out, uint32_t* keys_row_out, const uint32_t* keys_in) {
     if (n <= 0) {
         if (order == kRowMajor) return launch_fill_u32(out.cell_start, 0u, total_cells + 1, s);
         return 0;
@@ -466,16 +468,15 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, c
         const bool last = pass == passes - 1;
         uint32_t* vdst = last ? out.perm : sb.vals[cur ^ 1];
         const uint32_t* vsrc = pass == 0 ? (const uint32_t*)nullptr : sb.vals[cur];
-        double* inv = (last && hpos_to_rows) ? rows.p : (double*)nullptr;
         const int add_off = (last && order == kRowMajor) ? 1 : 0;
         if (tile == kSortTileLarge)
             hipLaunchKernelGGL((k_radix_scatter<kSortTileLarge, 8>), dim3(ntiles), dim3(512), 0, s, kin, vsrc,
                                sb.keys[cur ^ 1], vdst, geom, st.first, nmocks, side, shift, add_off, sb.tile_counts,
-                               sb.digit_totals, ntiles, inv, rows.stride);
+                               sb.digit_totals, ntiles);
         else
             hipLaunchKernelGGL((k_radix_scatter<kSortTile, 4>), dim3(ntiles), dim3(256), 0, s, kin, vsrc,
                                sb.keys[cur ^ 1], vdst, geom, st.first, nmocks, side, shift, add_off, sb.tile_counts,
-                               sb.digit_totals, ntiles, inv, rows.stride);
+                               sb.digit_totals, ntiles);
         cur ^= 1;
     }
     if (check("sort")) return -1;
@@ -484,7 +485,7 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, c
     if (order == kRowMajor) {
         if (!rows.p) return -1;  // the row-major gather reads the packed rows
         hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, s, out.perm, n, rows, w, nw, out.x, out.y, out.z,
-                           out.w, hrank_out);
+                           out.w, hkey_out);
     }
     if (order == kRowMajor) {
         hipLaunchKernelGGL(k_cell_start, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, sb.keys[cur], n,

@@ -111,11 +111,12 @@ int launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 int launch_group_mock(const MockGeom* g, int nmocks, int side, int32_t* group_mock,
                       int64_t ngroups_total, hipStream_t s);
 // The side's key pass (the first build of the side) writes its packed rows
-// (`rows`, when rows.p is set and keys_in is null).  Symmetric auto-counts
-// also need, for every point of the row-major order, its position in the
-// Hilbert order: the Hilbert build's last radix pass writes that position
-// into the point's row (hpos_to_rows), the row-major gather reads it through
-// its permutation (hrank_out).  When one side is sorted in both orders, the
+// (`rows`, when rows.p is set and keys_in is null; a Hilbert key pass puts
+// each point's Hilbert key in its row).  Symmetric auto-counts also need, for
+// every point of the row-major order, its place in the Hilbert order: the
+// row-major gather reads the key through its permutation and writes
+// (Hilbert key << 32) | input index (hkey_out), which orders the points of a
+// mock as the stable Hilbert sort does.  When one side is sorted in both orders, the
 // Hilbert build also writes the row-major keys (keys_row_out) and the
 // row-major build starts from them (keys_in).  Null pointers skip each of
 // these.
@@ -126,7 +127,7 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order,
                       const double* x, const double* y, const double* z, const double* w,
                       int nw, int64_t n, int64_t total_cells, int key_bits,
                       SortBufs sb, SortTiles st, SortedSide out, int32_t* status, hipStream_t s, PackedRows rows,
-                      bool hpos_to_rows = false, uint32_t* hrank_out = nullptr, uint32_t* keys_row_out = nullptr,
+                      uint64_t* hkey_out = nullptr, uint32_t* keys_row_out = nullptr,
                       const uint32_t* keys_in = nullptr);
 
 // pairs.hip
@@ -153,9 +154,14 @@ struct PairLaunch {
     double* rows;          // [n_lane_total][nr] (original order) or nullptr
     double* corr;          // auto-counts: rows2 - rows1 corrections from seam pairs, or nullptr
     uint64_t* pevals;      // [ngroups]
-    // auto-counts (unweighted): Hilbert position of each stream point; each
-    // unshifted unordered pair is then evaluated once and counted twice
-    const uint32_t* hrank;
+    // symmetric auto-counts: the Hilbert order key of each stream point,
+    // (Hilbert key << 32) | input index; each unshifted unordered pair is then
+    // evaluated once and counted twice.  hrows/hrow_stride: the lane side's
+    // packed rows (the Hilbert key of each point in its fourth word), for the
+    // key of the point that follows a lane group
+    const uint64_t* hkey;
+    const double* hrows;
+    int hrow_stride;
     // jackknife modes: per-mock subsample tables [nmocks][nsamp + 1][nbins]
     // (row 0: all pairs; row s: weight lost when subsample s is removed),
     // accumulated with atomics; lane and stream weights are (w, tag) rows

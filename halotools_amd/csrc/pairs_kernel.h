@@ -535,8 +535,8 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
         // group in the lane (Hilbert) order and count twice; the group's own
         // points are paired separately below; shifted images (whose two orders
         // may round apart) are evaluated from both sides, once each.
-        const uint32_t* __restrict__ HR = L.hrank;
-        const bool sym = (!F64 || MODE == kModeTan || JK) && !SWAP && HR != nullptr;
+        const uint64_t* __restrict__ HK = L.hkey;
+        const bool sym = (!F64 || MODE == kModeTan || JK) && !SWAP && HK != nullptr;
 
         // ---- jackknife tables.  Row 0 sums w1 w2 of every pair; the row of tag t
         // sums what removing subsample t takes away from it: w1 w2 when both
@@ -646,7 +646,17 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                     jk_global(ts, a, sv);
             }
         };
-        const uint32_t hend = (uint32_t)(p0 + cnt);
+        // symmetric auto-counts: a stream point follows the group in the Hilbert
+        // order iff its order key, (Hilbert key << 32) | input index, is at
+        // least that of the point at Hilbert position p0 + cnt (none after the
+        // mock's last group)
+        uint64_t hend = ~0ull;
+        if (sym && p0 + cnt < G.p_off[0] + G.n[0]) {
+            const uint32_t pn = sload_u32_at(L.lane.perm, p0 + cnt);
+            const uint32_t hk = sload_u32_at(reinterpret_cast<const uint32_t*>(L.hrows),
+                                             (int64_t)pn * (2 * L.hrow_stride) + 6);
+            hend = ((uint64_t)hk << 32) | (uint64_t)pn;
+        }
         // r: mock-relative stream index
         // inc: count-mode increment (2 for an unshifted pair evaluated once for both orders)
         auto accumulate = [&](uint32_t a, uint32_t r, uint32_t inc, bool) __attribute__((always_inline)) {
@@ -972,7 +982,7 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                             const float gz = fmaxf(0.f, fmaxf(fz - bzhi, bzlo - fz));
                             const float gxy = gx * gx + gy * gy;
                             ok = (shape == 0) ? (gxy + gz * gz <= r2xy) : (gxy <= r2xy && gz <= Rz);
-                            if (after_group) ok = ok && HR[jj] >= hend;
+                            if (after_group) ok = ok && HK[jj] >= hend;
                         }
                         const uint64_t m = (uint64_t)__ballot(ok);
                         if (ok) {

@@ -147,7 +147,7 @@ int hp_npairs_3d(const hp_request* req);
  * a fixed HP_NOALLOC_SCRATCH_BYTES (64 MiB) array that every GPU the library
  * runs on holds from module load until the process ends.  A request fits when
  * hp_workspace_size(req) <= HP_NOALLOC_SCRATCH_BYTES (an auto-count of about
- * 750K points with 20 bins); a larger one returns an error.  The current HIP
+ * 720K points with 20 bins); a larger one returns an error.  The current HIP
  * device's array is used (the stream must belong to that device), and it keeps
  * the last call's contents: calls that overlap on one GPU race. */
 int hp_npairs_3d_noalloc(const hp_request* req);
