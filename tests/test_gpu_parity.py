@@ -181,6 +181,24 @@ def test_zero_edges_and_duplicates(mo):
     assert np.all(mo.npairs_xy_z(s, s, rb, pi, period=1.0) == ORC.npairs_xy_z(s, s, rb, pi, period=1.0))
 
 
+def test_hilbert_key_ties_across_groups(mo):
+    # symmetric auto-counts keep a stream point when its (Hilbert key, input
+    # index) follows that of the point after the lane group: 60 copies of each
+    # of 40 points share one Hilbert key and straddle lane-group boundaries of
+    # 64, in shuffled input order, so the input index decides every tie
+    rng = np.random.default_rng(21)
+    s = np.concatenate([np.repeat(random_sample(40, seed=21), 60, axis=0), random_sample(3000, seed=22)])
+    s = s[rng.permutation(len(s))]
+    rb = np.array([0.0, 0.001, 0.05, 0.2])
+    for per in (1.0, None):
+        assert np.all(mo.npairs_3d(s, s, rb, period=per) == ORC.npairs_3d(s, s, rb, period=per))
+    pi = np.array([0.0, 0.05, 0.1])
+    assert np.all(mo.npairs_xy_z(s, s, rb, pi, period=1.0) == ORC.npairs_xy_z(s, s, rb, pi, period=1.0))
+    t = 1 + (np.minimum((s * 3).astype(int), 2) * np.array([9, 3, 1])).sum(axis=1)
+    got = mo.npairs_jackknife_3d(s, s, rb, t, t, 27, period=1.0)
+    assert np.array_equal(got, ORC.npairs_jackknife(s, s, rb, t, t, 27, period=1.0, num_threads=8))
+
+
 def test_clustered_medium(mo):
     # clustered auto-count: tight clumps make dense cells and many small separations
     rng = np.random.default_rng(123)
