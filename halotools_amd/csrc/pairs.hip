@@ -26,7 +26,6 @@ static int wave_layout(const PairLaunch& L, int kc, int jslots, PairArgs& A) {
     const int Kp = L.bp.nbins;
     A.shared_count = 0;
     A.cnt_off = 0;
-    A.tan_off = 0;
     A.jrow = 0;
     A.jr0_off = 0;
     switch (L.mode) {
@@ -35,7 +34,7 @@ static int wave_layout(const PairLaunch& L, int kc, int jslots, PairArgs& A) {
             if (L.br.nbins > 64) { A.shared_count = 1; return (kc + 1) * 4; }
             return (kc + 1) * 32 * 4;  // rows shared by lanes l and l+32
         case kModeXYZ: return (kc + 1) * (Kp + 1) * 4;
-        case kModeWXYZ: return HP_WXYZ_COMPACT ? kc * Kp * 8 : (kc + 1) * (Kp + 1) * 8;
+        case kModeWXYZ: return kc * Kp * 8;  // only the cells pairs in range write
         case kModeJack:
         case kModeJackXY: {
             // row 0 plus one row per subsample tag the wave keeps in LDS
@@ -56,16 +55,11 @@ static int wave_layout(const PairLaunch& L, int kc, int jslots, PairArgs& A) {
             A.cnt_off = (kc + 1) * 16 * 8;
             return A.cnt_off + (L.also_count ? (kc + 1) * 32 * 4 : 0);
         case kModeTan:
-            if (HP_TAN_INTERLEAVE) {
-                // per bin one block: W row, dW row (16 columns of 8 B each), then
-                // the count row (32 columns of 4 B) when counts are wanted
-                A.tan_off = 128;
-                A.cnt_off = 256;
-                return (kc + 1) * (L.also_count ? 384 : 256);
-            }
-            A.tan_off = (kc + 1) * 16 * 8;
-            A.cnt_off = 2 * A.tan_off;
-            return A.cnt_off + (L.also_count ? (kc + 1) * 32 * 4 : 0);
+            // per bin one block: W row, dW row (16 columns of 8 B each), then
+            // the count row (32 columns of 4 B) when counts are wanted (the
+            // kernel's kTanRow and kTanCnt)
+            A.cnt_off = 256;
+            return (kc + 1) * (L.also_count ? 384 : 256);
         default: return -1;
     }
 }

@@ -180,7 +180,6 @@ struct PairArgs {
     PairLaunch L;
     int wave_bytes;     // histogram bytes per wave
     int cnt_off;        // marked + counts: byte offset of the count rows
-    int tan_off;        // tangent mode: byte offset of the second FP64 rows
     int shared_count;   // 1: one histogram per wave instead of per lane
     int smem;           // dynamic LDS bytes
     int ring_off;       // byte offset of the survivor ring within a wave's LDS
@@ -213,15 +212,6 @@ constexpr int kWavesPerSimd =
 // Modes whose workgroups are persistent (measured on MI355X: the FP64-histogram
 // passes gain 2-7 %; the integer count passes lose 8-26 %, their register
 // budget leaving no room for the work loop's state without spilling).
-#ifndef HP_WXYZ_COMPACT
-#define HP_WXYZ_COMPACT 1  // marked (rp, pi) table without trash row and column
-#endif
-#ifndef HP_TAN_INTERLEAVE
-#define HP_TAN_INTERLEAVE 1  // forward-mode pass: a bin's W, dW and count rows in one block
-#endif
-#ifndef HP_TAN_FMA
-#define HP_TAN_FMA 1  // forward-mode pass: t_i w_j + w_i t_j with one fused multiply-add
-#endif
 #ifndef HP_SPREAD
 #define HP_SPREAD 1  // spread lane order for the modes with wave-shared histogram rows
 #endif
@@ -231,13 +221,9 @@ constexpr int kWavesPerSimd =
 #ifndef HP_XCD_CHUNK
 #define HP_XCD_CHUNK 64  // non-persistent modes: blocks per XCD run (0: blocks in dispatch order)
 #endif
-#ifndef HP_PERSIST_COUNT
-#define HP_PERSIST_COUNT 0
-#endif
 template <int MODE>
 constexpr bool kPersist = HP_PERSIST && (MODE == kModeWSum || MODE == kModeJac || MODE == kModeTan ||
-                                         MODE == kModeWXYZ || MODE == kModeJack || MODE == kModeJackXY ||
-                                         HP_PERSIST_COUNT);
+                                         MODE == kModeWXYZ || MODE == kModeJack || MODE == kModeJackXY);
 
 // VAR selects a variant of the mode: the Jacobian pass with the lanes on
 // sample 2 (kModeJac), or the product marking function w1[0] * w2[0] with one
@@ -296,12 +282,12 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
     // one 384-byte block (128 B each), so the second and third atomics of a
     // pair address the first one's block at fixed offsets (immediate offsets
     // of the LDS instructions, no address arithmetic).
-    constexpr bool TANI = (MODE == kModeTan) && HP_TAN_INTERLEAVE;
+    constexpr bool TANI = (MODE == kModeTan);
     constexpr uint32_t kTanRow = 128u, kTanCnt = 256u;
     // The marked (rp, pi) pass never writes its trash row or column (pairs
     // outside (rp_max, pi_max) skip the atomics), so its table has Kr x Kp
     // cells only: 20 waves per CU fit in the LDS instead of 16.
-    constexpr bool WXC = (MODE == kModeWXYZ) && HP_WXYZ_COMPACT;
+    constexpr bool WXC = (MODE == kModeWXYZ);
     const uint32_t step_r =
         XY ? (uint32_t)(WXC ? Kp : Kp + 1) * cell_bytes
            : (JK ? 8u
@@ -839,14 +825,14 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                         const double v = rl0 * bw[u];
                         // one fused multiply-add (a tangent term, not a distance:
                         // within the sums' 1e-12 tolerance, deterministic)
-                        const double tv = HP_TAN_FMA ? __builtin_fma(rl0, bt[u], rl1 * bw[u]) : rl1 * bw[u] + rl0 * bt[u];
+                        const double tv = __builtin_fma(rl0, bt[u], rl1 * bw[u]);
                         __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a[u]), v, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
-                        __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + (TANI ? kTanRow : A.tan_off) + a[u]), tv,
+                        __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + kTanRow + a[u]), tv,
                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         if (ALSO)
                             __hip_atomic_fetch_add(
-                                reinterpret_cast<uint32_t*>(wsm + (TANI ? kTanCnt : A.cnt_off) + a[u] + cnt_col), rinc,
+                                reinterpret_cast<uint32_t*>(wsm + kTanCnt + a[u] + cnt_col), rinc,
                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     } else {
                         __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a[u]), bw[u], __ATOMIC_RELAXED,
@@ -1056,14 +1042,14 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                         const double h = (kb + u > lane) ? 1.0 : 0.5;
                         const double wj = bw[u] * h, tj = bt[u] * h;
                         const double v = lw[0] * wj;
-                        const double tv = HP_TAN_FMA ? __builtin_fma(lw[0], tj, lw[1] * wj) : lw[1] * wj + lw[0] * tj;
+                        const double tv = __builtin_fma(lw[0], tj, lw[1] * wj);
                         __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + a[u]), v, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
-                        __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + (TANI ? kTanRow : A.tan_off) + a[u]), tv,
+                        __hip_atomic_fetch_add(reinterpret_cast<double*>(hl + kTanRow + a[u]), tv,
                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         if (ALSO) {
                             __hip_atomic_fetch_add(
-                                reinterpret_cast<uint32_t*>(wsm + (TANI ? kTanCnt : A.cnt_off) + a[u] + cnt_col), inc,
+                                reinterpret_cast<uint32_t*>(wsm + kTanCnt + a[u] + cnt_col), inc,
                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         }
                     } else if (MODE != kModeTan && (!XY || a[u] != trash_r)) {
@@ -1174,10 +1160,10 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
         }
         if (MODE == kModeTan) {
             // rows of w_i w_j and of t_i w_j + w_i t_j, doubled under symmetry
-            const double* th = reinterpret_cast<const double*>(wsm + (TANI ? kTanRow : A.tan_off));
+            const double* th = reinterpret_cast<const double*>(wsm + kTanRow);
             const double sc = sym ? 2.0 : 1.0;
             for (int s = 0; s < kc; s++) {
-                const int rs = TANI ? (ALSO ? 48 : 32) : 16;  // doubles from one bin's W row to the next
+                const int rs = ALSO ? 48 : 32;  // doubles from one bin's W row to the next
                 const double ws = wave_sum_f64(lane < 16 ? fh[s * rs + lane] : 0.0);
                 const double ts = wave_sum_f64(lane < 16 ? th[s * rs + lane] : 0.0);
                 if (lane == 0) {
