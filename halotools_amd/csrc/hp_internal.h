@@ -116,9 +116,9 @@ int launch_group_mock(const MockGeom* g, int nmocks, int side, int32_t* group_mo
 // every point of the row-major order, its place in the Hilbert order: the
 // row-major gather reads the key through its permutation and writes
 // (Hilbert key << 32) | input index (hkey_out), which orders the points of a
-// mock as the stable Hilbert sort does.  When one side is sorted in both orders, the
-// Hilbert build also writes the row-major keys (keys_row_out) and the
-// row-major build starts from them (keys_in).  Null pointers skip each of
+// mock as the stable Hilbert sort does.  When one side is sorted in both
+// orders, the Hilbert build also writes the row-major keys (keys_row_out) and
+// the row-major build starts from them (keys_in).  Null pointers skip each of
 // these.
 // Keys per sort tile for a sort of n keys (HALOPAIR_SORT_TILE = 2048 or 8192
 // forces one size, for tests and measurements).
