@@ -482,6 +482,52 @@ def bench_build64(dev, steps):
     return _build_rate(int(sizes.sum()), build_s)
 
 
+def bench_host_phases(dev, t, rbins, L, calls=60):
+    """Host time per phase of the headline drop-in call (median over `calls`):
+    the argument checks, the column split of the (N, 3) tensor, the request
+    build, the library call (planning and kernel enqueues) and the wait for
+    the status word -- the same steps as npairs_3d, timed one by one."""
+    import numpy as np
+    import torch
+
+    from halotools_amd import _device as D
+    from halotools_amd import _engine as E
+    from halotools_amd.mock_observables.pair_counters import _common as C
+    from halotools_amd.mock_observables.pair_counters import _process as P
+    names = ("argument_checks", "column_split", "request_build", "library_call", "wait", "total")
+    ph = {k: [] for k in names}
+    torch.cuda.synchronize()
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        x1, y1, z1 = P.columns(t)
+        x2, y2, z2 = P.columns(t)
+        rb = np.atleast_1d(rbins).astype("f8")
+        rmax = np.max(rb)
+        rb = P.process_bins(rb, P.RBINS_MSG)
+        P.check_last_edge(rb)
+        period = P.process_period(L)
+        a1, a2 = P.default_cell_sizes(None, None, (rmax, rmax, rmax))
+        P.set_approximate_cell_sizes(a1, a2, period)
+        P.check_search_lengths((rmax, rmax, rmax), tuple(period))
+        device = D.require_gpu(t.device)
+        same = D.same_array(t, t)
+        t1 = time.perf_counter()
+        s1, s2 = C.upload((x1, y1, z1), (x2, y2, z2), same, device)
+        mock = C.mock_dict(len(s1[0]), len(s2[0]), period, True)
+        t2 = time.perf_counter()
+        res = D.prepare(E.HP_NPAIRS_3D, [mock], s1, s2, P.squared(rb), same=same, device=device)
+        t3 = time.perf_counter()
+        D.launch(res)
+        t4 = time.perf_counter()
+        C.finish(res, lambda host: res.counts[0], True, period, True)
+        t5 = time.perf_counter()
+        for k, v in zip(names, (t1 - t0, t2 - t1, t3 - t2, t4 - t3, t5 - t4, t5 - t0)):
+            ph[k].append(v * 1e6)
+    out = {k + "_us": float(np.median(v)) for k, v in ph.items()}
+    _progress("host phases: " + ", ".join("%s %.1f us" % (k, v) for k, v in out.items()))
+    return out
+
+
 def _build_rate(npts, seconds):
     """Cell build of one auto-count sample against the HBM roofline.
 
@@ -506,7 +552,8 @@ def parse():
     ap.add_argument("--no-others", action="store_true", help="skip the xy_z / projected / marked timings")
     ap.add_argument("--legs", default="all",
                     help="comma list of legs to run besides the headline: batched, " + ", ".join(OTHER_LEGS) +
-                         "; build64 (the batch's cell build alone) runs only when named")
+                         "; build64 (the batch's cell build alone) and hostphases (host time per phase of "
+                         "the headline call) run only when named")
     return ap.parse_args()
 
 
@@ -645,6 +692,7 @@ def main():
     legs = set(["batched"] + list(OTHER_LEGS)) if args.legs == "all" else set(args.legs.split(","))
     batched = None if (args.no_batched or "batched" not in legs) else bench_batched(dev, max(3, args.steps // 4))
     build64 = bench_build64(dev, max(3, args.steps // 4)) if "build64" in legs else None
+    host_phases = bench_host_phases(dev, t, rbins, L) if "hostphases" in legs else None
     other = [g for g in OTHER_LEGS if g in legs]
     others = None if (args.no_others or not other) else bench_other_configs(
         dev, max(3, args.steps // 4), cpu_threads=0 if args.no_cpu_baseline else threads, legs=other)
@@ -682,6 +730,7 @@ def main():
                                        "(profiles/k_pairs_count3d_clustered_1M.md)"},
         "batched_64_mocks": batched,
         "cell_build_64_mocks": build64,
+        "headline_host_phases": host_phases,
         "other_configs": others,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
