@@ -1,6 +1,7 @@
 """GPU parity at the benchmark's full sizes.
 
-The ~1M-point NFW-clustered box (npairs_3d, per-object rows, jackknife table),
+The 20k x 20k cross-count in an L = 250 box (the reference's CPU-runnable
+case, bit-exact), the ~1M-point NFW-clustered box (npairs_3d, per-object rows, jackknife table),
 2M uniform points (npairs_xy_z with 16 x 41 bins and npairs_projected),
 1M uniform halos with Zheng07 weights (product-marked sums and their gradient
 with respect to (logMmin, sigma_logM)), and the 64-mock batch of 50k-2M
@@ -26,6 +27,18 @@ from oracle import mesh as ORC
 pytestmark = pytest.mark.gpu
 
 THREADS = min(os.cpu_count() or 1, 16)
+
+
+def test_c1_cross_20k_exact(gpu):
+    # the reference's CPU-runnable case: 20k x 20k cross-count in a periodic
+    # L = 250 box, 15 log edges 0.1-20; both orders and the non-periodic box
+    from halotools_amd import mocks
+    from halotools_amd.mock_observables import npairs_3d
+    a, b, L = mocks.config_points(1)
+    rb = mocks.config_rbins(1)
+    for s1, s2 in ((a, b), (b, a)):
+        assert np.array_equal(npairs_3d(s1, s2, rb, period=L), ORC.npairs_3d(s1, s2, rb, period=L))
+    assert np.array_equal(npairs_3d(a, b, rb), ORC.npairs_3d(a, b, rb))
 
 
 @pytest.fixture(scope="module")
