@@ -36,6 +36,10 @@ def array_is_monotonic(array, strict=False):
     """halotools/utils/array_utils.py:135-186."""
     if custom_len(array) < 3:
         raise HalotoolsError("Input array to the array_is_monotonic method has less then 3 elements")
+    if strict is True and isinstance(array, np.ndarray) and array.ndim == 1:
+        # the common case of a checked bin array, in one comparison pass
+        if bool((array[1:] > array[:-1]).all()):
+            return 1
     d = np.diff(array)
     if strict is True:
         if np.all(d > 0):
@@ -85,7 +89,8 @@ def check_last_edge(bins):
     beyond the search length would count only where its mesh happens to
     reach, so the result is not defined by the inputs.
     """
-    if bins[-1] * bins[-1] > np.max(bins) * np.max(bins):
+    m = bins.max()
+    if bins[-1] * bins[-1] > m * m:
         raise ValueError("Input bins whose last edge squared exceeds max(bins)**2 are not supported: "
                          "the reference's result for them depends on its mesh")
 
@@ -98,6 +103,13 @@ PI_BINS_MSG = "Input ``pi_bins`` must be a monotonically increasing 1D array wit
 
 def process_period(period):
     """PC/npairs_3d.py:190-200."""
+    if type(period) in (float, int):
+        # a Python scalar: the same array and checks without NumPy's
+        # conversions of a one-element array
+        p = float(period)
+        if not (p < np.inf and p > 0):
+            raise ValueError("Input ``period`` must be a bounded positive number in all dimensions")
+        return np.array([p, p, p])
     period = np.atleast_1d(period).astype(float)
     if len(period) == 1:
         period = np.array([period[0]] * 3)
@@ -196,6 +208,8 @@ def check_search_lengths(search, period):
 
 def columns(sample):
     """``sample[:, 0]`` ... as the reference reads them (PC/npairs_3d.py:165-170)."""
+    if isinstance(sample, torch.Tensor) and sample.dim() == 2 and sample.shape[1] == 3:
+        return sample.unbind(1)  # the same three column views in one call
     return sample[:, 0], sample[:, 1], sample[:, 2]
 
 
