@@ -386,11 +386,21 @@ __global__ void __launch_bounds__(WAVES * 64) k_radix_scatter(const uint32_t* __
 // the row) read at its input position.  hkey: the Hilbert order key of each
 // point, (Hilbert key << 32) | input index -- the stable Hilbert sort orders
 // the points of a mock by exactly this value.
+// With `keys` (the sorted global cell ids) the same pass writes the cell
+// starts: cell_start[c] = first sorted position whose key >= c, for c in
+// [0, total_cells]; thread q = n writes the trailing empty cells.
 __global__ void k_gather(const uint32_t* __restrict__ perm, int64_t n, PackedRows rows, const double* __restrict__ w,
                          int nw, double* __restrict__ ox, double* __restrict__ oy, double* __restrict__ oz,
-                         double* __restrict__ ow, uint64_t* __restrict__ hkey) {
+                         double* __restrict__ ow, uint64_t* __restrict__ hkey, const uint32_t* __restrict__ keys,
+                         int64_t total_cells, uint32_t* __restrict__ cell_start) {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q >= n) return;
+    if (q > n) return;
+    if (keys) {
+        const int64_t kcur = (q < n) ? (int64_t)keys[q] : total_cells;
+        const int64_t kprev = (q > 0) ? (int64_t)keys[q - 1] : -1;
+        for (int64_t c = kprev + 1; c <= kcur && c <= total_cells; c++) cell_start[c] = (uint32_t)q;
+    }
+    if (q == n) return;
     const int64_t p = perm[q];
     const double* r = rows.p + p * rows.stride;
     const double4 c = reinterpret_cast<const double4*>(r)[0];
@@ -409,16 +419,6 @@ __global__ void k_gather(const uint32_t* __restrict__ perm, int64_t n, PackedRow
             for (int k = 0; k < nw; k++) ow[q * nw + k] = w[p * nw + k];
         }
     }
-}
-
-// cell_start[c] = first sorted position whose key >= c, for c in [0, total].
-__global__ void k_cell_start(const uint32_t* __restrict__ keys, int64_t n, int64_t total_cells,
-                             uint32_t* __restrict__ cell_start) {
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (q > n) return;
-    const int64_t kcur = (q < n) ? (int64_t)keys[q] : total_cells;
-    const int64_t kprev = (q > 0) ? (int64_t)keys[q - 1] : -1;
-    for (int64_t c = kprev + 1; c <= kcur && c <= total_cells; c++) cell_start[c] = (uint32_t)q;
 }
 
 static inline int check(const char*) { return hipGetLastError() == hipSuccess ? 0 : -1; }
@@ -483,13 +483,10 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, c
     // The lane side (Hilbert order) is read once per group through the
     // permutation; only the stream side needs sorted copies.
     if (order == kRowMajor) {
+        // the gather of the packed rows and the cell starts in one pass
         if (!rows.p) return -1;  // the row-major gather reads the packed rows
-        hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, s, out.perm, n, rows, w, nw, out.x, out.y, out.z,
-                           out.w, hkey_out);
-    }
-    if (order == kRowMajor) {
-        hipLaunchKernelGGL(k_cell_start, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, sb.keys[cur], n,
-                           total_cells, out.cell_start);
+        hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, out.perm, n, rows, w,
+                           nw, out.x, out.y, out.z, out.w, hkey_out, sb.keys[cur], total_cells, out.cell_start);
     }
     return check("gather");
 }
