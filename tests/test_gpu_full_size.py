@@ -26,7 +26,14 @@ from oracle import mesh as ORC
 
 pytestmark = pytest.mark.gpu
 
-THREADS = min(os.cpu_count() or 1, 16)
+# oracle threads: the CPUs this process may run on, as the bench's CPU leg
+# takes them (its affinity set, capped by OMP_NUM_THREADS when set)
+try:
+    THREADS = len(os.sched_getaffinity(0))
+except (AttributeError, OSError):
+    THREADS = os.cpu_count() or 1
+if os.environ.get("OMP_NUM_THREADS", "").isdigit() and int(os.environ["OMP_NUM_THREADS"]) > 0:
+    THREADS = min(THREADS, int(os.environ["OMP_NUM_THREADS"]))
 
 
 def test_c1_cross_20k_exact(gpu):
