@@ -61,39 +61,51 @@ def xyz_rows(cols):
 
 def tensor_xyz_rows(cols):
     """The float64 (N, 3) tensor whose columns ``cols`` are (as a view), or None."""
-    if not all(isinstance(c, torch.Tensor) for c in cols):
+    if len(cols) != 3:
         return None
-    c0 = cols[0]
-    if c0.dim() != 1 or c0.shape[0] == 0:
+    c0, c1, c2 = cols
+    if not (isinstance(c0, torch.Tensor) and isinstance(c1, torch.Tensor) and isinstance(c2, torch.Tensor)):
+        return None
+    f64 = torch.float64
+    if c0.dtype != f64 or c1.dtype != f64 or c2.dtype != f64:
+        return None
+    # a stride of (3,) also means one dimension
+    if c0.stride() != (3,) or c1.stride() != (3,) or c2.stride() != (3,):
         return None
     n = c0.shape[0]
-    if any(c.dtype != torch.float64 or c.dim() != 1 or c.shape[0] != n or c.device != c0.device for c in cols):
-        return None
-    if any(c.stride() != (3,) for c in cols):
+    if n == 0 or c1.shape[0] != n or c2.shape[0] != n or c1.device != c0.device or c2.device != c0.device:
         return None
     p0 = c0.data_ptr()
-    if any(cols[k].data_ptr() != p0 + 8 * k for k in range(3)):
+    if c1.data_ptr() != p0 + 8 or c2.data_ptr() != p0 + 16:
         return None
     return torch.as_strided(c0, (n, 3), (3, 1))
 
 
-def _split_rows(rows):
-    """(N, 3) float64 device rows -> contiguous x, y, z (one transposing copy)."""
-    t = rows.t().contiguous()
-    return t[0], t[1], t[2]
+class RowColumns(tuple):
+    """The x, y, z column views of one (N, 3) row-major float64 tensor."""
+    stride = 3
+
+
+def _row_columns(rows):
+    """The x, y, z column views of (N, 3) float64 device rows (the library reads
+    them in place with a stride of 3 doubles)."""
+    return RowColumns(rows.unbind(1))
 
 
 def to_device_columns(cols, device):
-    """Three coordinate columns (NumPy or torch, any real dtype) -> contiguous float64 device tensors."""
+    """Three coordinate columns (NumPy or torch, any real dtype) -> float64 device
+    tensors: contiguous columns, or the three column views of one (N, 3) array."""
     if all(isinstance(c, torch.Tensor) for c in cols):
         rows = tensor_xyz_rows(cols)
         if rows is not None:
-            return _split_rows(rows.to(device))
+            if rows.device == device:
+                return RowColumns(cols)  # already the column views of rows on the device
+            return _row_columns(rows.to(device))
         return tuple(c.to(device=device, dtype=torch.float64).contiguous() for c in cols)
     base = xyz_rows(cols)
     if base is not None:
-        # one copy of the caller's (N, 3) rows; the columns are split on the device
-        return _split_rows(torch.from_numpy(base).to(device, non_blocking=False))
+        # one copy of the caller's (N, 3) rows, read in place on the device
+        return _row_columns(torch.from_numpy(base).to(device, non_blocking=False))
     host = np.empty((3, len(cols[0])), dtype=np.float64)
     for k in range(3):
         host[k] = np.asarray(cols[k])
@@ -105,6 +117,19 @@ def to_device_weights(w, device):
     if isinstance(w, torch.Tensor):
         return w.to(device=device, dtype=torch.float64).contiguous()
     return torch.from_numpy(np.ascontiguousarray(w, dtype=np.float64)).to(device)
+
+
+def _coordinate_columns(cols):
+    """(columns, stride) for a request: contiguous columns (stride 1), the
+    column views of one (N, 3) row-major tensor (stride 3), or contiguous
+    copies of anything else."""
+    if isinstance(cols, RowColumns):
+        return cols, 3
+    if all(c.is_contiguous() for c in cols):
+        return cols, 1
+    if tensor_xyz_rows(cols) is not None:
+        return cols, 3
+    return tuple(c.contiguous() for c in cols), 1
 
 
 def _ptr(t):
@@ -186,8 +211,10 @@ def prepare(kind, mocks, s1, s2, r2, p2=None, same=False, w1=None, w2=None, nw=0
     req.nmocks = M
     req.wfunc = int(wfunc)
     req.mocks = ctypes.cast(marr, ctypes.POINTER(E.HpMock))
+    s1, req.stride1 = _coordinate_columns(s1)
     req.x1, req.y1, req.z1 = [c.data_ptr() for c in s1]
     if not same:
+        s2, req.stride2 = _coordinate_columns(s2)
         req.x2, req.y2, req.z2 = [c.data_ptr() for c in s2]
     req.w1 = w1.data_ptr() if w1 is not None else None
     req.w2 = w2.data_ptr() if (w2 is not None and not same) else None

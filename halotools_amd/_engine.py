@@ -81,6 +81,8 @@ class HpRequest(ctypes.Structure):
         ("jtags1", ctypes.c_void_p),
         ("jtags2", ctypes.c_void_p),
         ("n_samples", ctypes.c_int32),
+        ("stride1", ctypes.c_int32),
+        ("stride2", ctypes.c_int32),
         ("reserved2", ctypes.c_int32),
     ]
 

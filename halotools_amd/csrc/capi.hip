@@ -479,13 +479,16 @@ static SortedSide side_ptrs(char* ws, const Side& S) {
     o.w = S.w ? reinterpret_cast<double*>(ws + S.w) : nullptr;
     o.perm = reinterpret_cast<uint32_t*>(ws + S.perm);
     o.cell_start = S.cs ? reinterpret_cast<uint32_t*>(ws + S.cs) : nullptr;
+    o.xs = 1;
     return o;
 }
 
-// Lane side: the input arrays, read through the Hilbert permutation.
+// Lane side: the input arrays (point i at x[i * xs] ...), read through the
+// Hilbert permutation.
 static SortedSide lane_ptrs(char* ws, const Side& S, const double* x, const double* y, const double* z,
-                            const double* w) {
+                            const double* w, int64_t xs) {
     SortedSide o = side_ptrs(ws, S);
+    o.xs = xs;
     o.x = const_cast<double*>(x);
     o.y = const_cast<double*>(y);
     o.z = const_cast<double*>(z);
@@ -519,6 +522,7 @@ static int run(const hp_request* r, int kind) {
     if (!r->workspace) return fail("workspace must not be null");
     if (r->workspace_bytes < P.total) return fail("workspace too small");
     if (P.n[0] > 0 && (!r->x1 || !r->y1 || !r->z1)) return fail("sample1 coordinates must not be null");
+    if (r->stride1 < 0 || r->stride2 < 0) return fail("coordinate strides must not be negative");
     if (!P.same && P.n[1] > 0 && (!r->x2 || !r->y2 || !r->z2))
         return fail("sample2 coordinates must not be null");
     if (P.marked && ((P.n[0] > 0 && !r->w1) || (!P.same && P.n[1] > 0 && !r->w2)))
@@ -540,6 +544,9 @@ static int run(const hp_request* r, int kind) {
     launch_fill_u32(reinterpret_cast<uint32_t*>(status), 0u, 1, s);
 
     const double* X[2] = {r->x1, P.same ? r->x1 : r->x2};
+    // doubles between consecutive points of each sample's coordinate arrays
+    const int64_t XS[2] = {r->stride1 > 1 ? r->stride1 : 1,
+                           P.same ? (r->stride1 > 1 ? r->stride1 : 1) : (r->stride2 > 1 ? r->stride2 : 1)};
     const double* Y[2] = {r->y1, P.same ? r->y1 : r->y2};
     const double* Z[2] = {r->z1, P.same ? r->z1 : r->z2};
     const double* W[2] = {r->w1, P.same ? r->w1 : r->w2};
@@ -591,7 +598,7 @@ static int run(const hp_request* r, int kind) {
                                   P.n[sd], P.total_cells, o == kRowMajor ? P.bits_row : P.bits_hilbert, sb, st,
                                   side_ptrs(ws, S), status, s, rows,
                                   o == kRowMajor ? hkey : nullptr, o == kHilbert ? krow : nullptr,
-                                  o == kRowMajor ? krow : nullptr))
+                                  o == kRowMajor ? krow : nullptr, XS[sd]))
                 return fail("cell build launch failed");
         }
     }
@@ -608,7 +615,7 @@ static int run(const hp_request* r, int kind) {
     L.geom = dgeom;
     L.group_mock = gm0;
     L.ngroups = P.ngroups[0];
-    L.lane = lane_ptrs(ws, P.sorted[0][kHilbert], X[0], Y[0], Z[0], P.weighted() ? W[0] : nullptr);
+    L.lane = lane_ptrs(ws, P.sorted[0][kHilbert], X[0], Y[0], Z[0], P.weighted() ? W[0] : nullptr, XS[0]);
     L.strm = side_ptrs(ws, P.sorted[s2][kRowMajor]);
     L.nw = P.nw;
     L.br = P.tr.spec;
@@ -683,7 +690,7 @@ static int run(const hp_request* r, int kind) {
         B.also_count = 0;
         B.group_mock = gm1;
         B.ngroups = P.ngroups[s2];
-        B.lane = lane_ptrs(ws, P.sorted[s2][kHilbert], X[s2], Y[s2], Z[s2], W[s2]);
+        B.lane = lane_ptrs(ws, P.sorted[s2][kHilbert], X[s2], Y[s2], Z[s2], W[s2], XS[s2]);
         B.strm = side_ptrs(ws, P.sorted[0][kRowMajor]);
         B.rows = r->rows2;
         B.pcount = nullptr;

@@ -103,7 +103,8 @@ __device__ __forceinline__ int cell_index(double v, double inv_cell, int n, bool
 __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, int order,
                        const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
                        int64_t n, uint32_t* __restrict__ keys, int32_t* __restrict__ status,
-                       uint32_t* __restrict__ keys_row, const double* __restrict__ w, int nw, PackedRows rows) {
+                       uint32_t* __restrict__ keys_row, const double* __restrict__ w, int nw, PackedRows rows,
+                       int64_t xs) {
     const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (p >= n) return;
     // the mock of the point: a wave's 64 consecutive points nearly always lie
@@ -114,7 +115,7 @@ __global__ void k_keys(const MockGeom* __restrict__ geom, int nmocks, int side, 
     const int64_t ml = mock_of_point(geom, nmocks, side, min((int64_t)pf + 63, n - 1));
     const int64_t m = (mf == ml) ? mf : mock_of_point(geom, nmocks, side, p);
     const MockGeom& G = geom[m];
-    const double px = x[p], py = y[p], pz = z[p];
+    const double px = x[p * xs], py = y[p * xs], pz = z[p * xs];
     bool bad = false;
     const int ix = cell_index(px, G.inv_cell[0], G.ndiv[0], bad, G.period[0], G.pbc);
     const int iy = cell_index(py, G.inv_cell[1], G.ndiv[1], bad, G.period[1], G.pbc);
@@ -437,7 +438,7 @@ int sort_tile(int64_t n) {
 int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, const double* x, const double* y,
                       const double* z, const double* w, int nw, int64_t n, int64_t total_cells, int key_bits,
                       SortBufs sb, SortTiles st, SortedSide out, int32_t* status, hipStream_t s, PackedRows rows,
-                      uint64_t* hkey_out, uint32_t* keys_row_out, const uint32_t* keys_in) {
+                      uint64_t* hkey_out, uint32_t* keys_row_out, const uint32_t* keys_in, int64_t xs) {
     if (n <= 0) {
         if (order == kRowMajor) return launch_fill_u32(out.cell_start, 0u, total_cells + 1, s);
         return 0;
@@ -446,7 +447,7 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, c
     // keys_in: this order's keys were written by the other order's key pass
     if (!keys_in)
         hipLaunchKernelGGL(k_keys, dim3(nb), dim3(256), 0, s, geom, nmocks, side, (int)order, x, y, z, n, sb.keys[0],
-                           status, order == kHilbert ? keys_row_out : (uint32_t*)nullptr, w, nw, rows);
+                           status, order == kHilbert ? keys_row_out : (uint32_t*)nullptr, w, nw, rows, xs);
     const int tile = st.tile, ntiles = st.ntiles;
     if (tile != kSortTile && tile != kSortTileLarge) return -1;
     if (ntiles <= 0) return -1;

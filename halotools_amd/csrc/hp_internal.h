@@ -89,6 +89,7 @@ struct SortedSide {
     double* w;          // (n, nw) row-major, or nullptr
     uint32_t* perm;     // sorted position -> original index
     uint32_t* cell_start;  // row-major order only: [total_cells + 1]
+    int64_t xs;         // doubles between consecutive points of x, y, z (1 for sorted copies)
 };
 
 // One side's coordinates (and up to four weights) packed as fixed-size rows
@@ -128,7 +129,7 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order,
                       int nw, int64_t n, int64_t total_cells, int key_bits,
                       SortBufs sb, SortTiles st, SortedSide out, int32_t* status, hipStream_t s, PackedRows rows,
                       uint64_t* hkey_out = nullptr, uint32_t* keys_row_out = nullptr,
-                      const uint32_t* keys_in = nullptr);
+                      const uint32_t* keys_in = nullptr, int64_t xs = 1);
 
 // pairs.hip
 struct PairLaunch {

@@ -399,9 +399,9 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
         const int64_t q = (int64_t)L.lane.perm[p];
         // inactive lanes carry NaN: every comparison fails, so they land in the trash row
         const double qnan = __builtin_nan("");
-        const double px = act ? L.lane.x[q] : qnan;
-        const double py = act ? L.lane.y[q] : qnan;
-        const double pz = act ? L.lane.z[q] : qnan;
+        const double px = act ? L.lane.x[q * L.lane.xs] : qnan;
+        const double py = act ? L.lane.y[q * L.lane.xs] : qnan;
+        const double pz = act ? L.lane.z[q * L.lane.xs] : qnan;
 
         const double Lx = G.period[0], Ly = G.period[1], Lz = G.period[2];
         const int nx = G.ndiv[0], ny = G.ndiv[1], nz = G.ndiv[2];
@@ -1008,9 +1008,9 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                 for (int u = 0; u < B; u++) {
                     const int64_t qk =
                         (int64_t)sload_u32_at(L.lane.perm, p0 + spread_pos(min(kb + u, cnt - 1), cnt));
-                    bx[u] = sload_f64_at(L.lane.x, qk);
-                    by[u] = sload_f64_at(L.lane.y, qk);
-                    bz[u] = sload_f64_at(L.lane.z, qk);
+                    bx[u] = sload_f64_at(L.lane.x, qk * L.lane.xs);
+                    by[u] = sload_f64_at(L.lane.y, qk * L.lane.xs);
+                    bz[u] = sload_f64_at(L.lane.z, qk * L.lane.xs);
                     if (MODE == kModeTan || JK) {
                         bw[u] = sload_f64_at(L.lane.w, 2 * qk);
                         bt[u] = sload_f64_at(L.lane.w, 2 * qk + 1);

@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 1
+#define HP_ABI_VERSION 2
 #define HP_MAX_BINS (1 << 20) /* edges per axis (and at most 2^26 histogram cells) */
 #define HP_NOALLOC_SCRATCH_BYTES (64u << 20) /* fixed scratch of hp_npairs_3d_noalloc, per GPU */
 
@@ -86,8 +86,8 @@ typedef struct hp_request {
     int32_t wfunc;             /* marked: weight_func_id 1..17 (xy_z: 1..15)           */
     const hp_mock* mocks;      /* host [nmocks]; sample points of mock m follow those  */
                                /* of mock m-1 in the coordinate arrays                 */
-    const double *x1, *y1, *z1; /* sample 1, SoA, n1 total                              */
-    const double *x2, *y2, *z2; /* sample 2, SoA, n2 total (= sample 1 if SAME_SAMPLES) */
+    const double *x1, *y1, *z1; /* sample 1, n1 total: point i at x1[i * stride1] ...   */
+    const double *x2, *y2, *z2; /* sample 2, n2 total (= sample 1 if SAME_SAMPLES)      */
     const double* w1;          /* marked: weights1, row-major (n1_total, nw)           */
     const double* w2;          /* marked: weights2, row-major (n2_total, nw)           */
     int32_t nw;                /* weights per point (id 1,2: 1; 3-11: 2; 12,13: 4; ...) */
@@ -122,6 +122,11 @@ typedef struct hp_request {
     const int32_t* jtags1;
     const int32_t* jtags2;     /* ignored with HP_SAME_SAMPLES                        */
     int32_t n_samples;
+    /* doubles between consecutive points of x1, y1, z1 (resp. x2, y2, z2): 0 or  */
+    /* 1 for separate contiguous arrays, 3 for the columns of one row-major      */
+    /* (N, 3) array (x1 = its start, y1 = x1 + 1, z1 = x1 + 2), read in place    */
+    int32_t stride1;
+    int32_t stride2;
     int32_t reserved2;
 } hp_request;
 

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(lib, name), name
         ctypes.cast(getattr(lib, name), ctypes.c_void_p)
-    assert lib.hp_abi_version() == 1
+    assert lib.hp_abi_version() == 2
 
 
 def test_declared_symbols_match_binding_list():
@@ -324,9 +324,16 @@ def test_tensor_xyz_rows_detects_one_block_of_rows():
     rows = D.tensor_xyz_rows(P.columns(t))
     assert rows is not None and tuple(rows.shape) == (500, 3) and torch.equal(rows, t)
     assert rows.data_ptr() == t.data_ptr()
-    x, y, z = D._split_rows(rows)
+    # the request reads such columns in place with a stride of 3 doubles
+    cols, stride = D._coordinate_columns(P.columns(t))
+    assert stride == 3 and [c.data_ptr() for c in cols] == [t.data_ptr() + 8 * k for k in range(3)]
+    x, y, z = D._row_columns(rows)
     assert torch.equal(x, t[:, 0]) and torch.equal(y, t[:, 1]) and torch.equal(z, t[:, 2])
-    assert x.is_contiguous() and z.is_contiguous()
+    # contiguous columns keep stride 1; other layouts are copied to contiguous columns
+    cols, stride = D._coordinate_columns(tuple(c.contiguous() for c in (x, y, z)))
+    assert stride == 1
+    cols, stride = D._coordinate_columns((t[:, 2], t[:, 1], t[:, 0]))
+    assert stride == 1 and all(c.is_contiguous() for c in cols) and torch.equal(cols[0], t[:, 2])
     assert D.tensor_xyz_rows(P.columns(t.float())) is None
     assert D.tensor_xyz_rows(P.columns(t[::2])) is None
     assert D.tensor_xyz_rows((t[:, 2], t[:, 1], t[:, 0])) is None
