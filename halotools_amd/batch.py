@@ -55,7 +55,10 @@ class MockBatch(object):
         self.sizes = [int(n) for n in sizes]
         self.M = len(self.sizes)
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
-        self.cols = (allp[:, 0].contiguous(), allp[:, 1].contiguous(), allp[:, 2].contiguous())
+        # the column views of one (N, 3) block: the library reads the rows in place
+        # (one 24-byte span per point in the key pass and the lane reads)
+        self.rows = allp.contiguous()
+        self.cols = D.RowColumns(self.rows.unbind(1))
         self.rbins = P.process_bins(rbins, P.RBINS_MSG)
         self.period = P.process_period(period)
         rmax = float(np.max(self.rbins))
@@ -102,7 +105,6 @@ class MockBatch(object):
         cache key is timed; every mock of the bucket then uses its result.
         Returns {bucket key: ((target, waves), timings)}.
         """
-        x, y, z = self.cols
         done = {}
         for m in np.argsort(self.sizes)[::-1]:
             n = self.sizes[m]
@@ -110,7 +112,7 @@ class MockBatch(object):
             if k in done or n < 64:
                 continue
             a, b = int(self.offsets[m]), int(self.offsets[m + 1])
-            pts = torch.stack((x[a:b], y[a:b], z[a:b]), dim=1)
+            pts = self.rows[a:b]
             done[k] = autotune.tune(pts, self.period, self.rbins, kind=kind, device=self.device, **kw)
         self._apply_tuning(kind)
         return done
