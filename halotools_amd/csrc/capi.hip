@@ -151,7 +151,6 @@ struct Plan {
     size_t o_pcount = 0, o_psum = 0, o_pevals = 0, o_red = 0, o_status = 0;
     bool elim = false;
     size_t o_corr = 0, o_rows_tmp = 0;
-    size_t o_hkey = 0;               // symmetric auto-counts: Hilbert order keys of the row-major order
     size_t o_rows[2] = {0, 0};       // packed coordinate rows of each side with a row-major order
     int row_stride[2] = {4, 4}, row_nw[2] = {0, 0};
     size_t o_keys_row = 0;           // row-major keys of a side sorted in both orders
@@ -331,9 +330,10 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
         max_cells = std::max<int64_t>(max_cells, (int64_t)ncells);
         max_hkeys = std::max<int64_t>(max_hkeys, (int64_t)1 << (3 * mb));
         if (nm[1] > 0 && (double)nm[1] * 27.0 >= 4.0e9) return fail("too many points in one mock for 32-bit histograms");
-        // the pair kernel addresses a mock's stream points (and weights) with 32-bit byte offsets
+        // the pair kernel addresses a mock's stream rows (32 bytes per point) and
+        // weights with 32-bit byte offsets
         for (int s = 0; s < 2; s++)
-            if ((double)nm[s] * 8.0 * (double)std::max(P.nw, 1) >= 4294967296.0)
+            if ((double)nm[s] * std::max(32.0, 8.0 * (double)std::max(P.nw, 1)) >= 4294967296.0)
                 return fail("too many points (times weights per point) in one mock for 32-bit offsets");
     }
     P.n[0] = p_off[0];
@@ -395,9 +395,11 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
             const size_t n = (size_t)std::max<int64_t>(P.n[s], 1);
             nmax = std::max<int64_t>(nmax, (int64_t)n);
             if (o == kRowMajor) {
-                S.x = take(n * 8);
-                S.y = take(n * 8);
-                S.z = take(n * 8);
+                // one 32-byte row per point: x, y, z and the Hilbert order key
+                // of the symmetric auto-count (else zero)
+                S.x = take(n * 32);
+                S.y = S.x + 8;
+                S.z = S.x + 16;
                 if (P.weighted()) S.w = take(n * 8 * (size_t)P.nw);
             }
             S.perm = take(n * 4);
@@ -436,7 +438,6 @@ static int make_plan(const hp_request* r, int kind, Plan& P) {
     // unweighted auto-counts evaluate each unshifted unordered pair once; they
     // need every row-major point's position in the Hilbert order
     P.sym = P.same && !P.obj && (!P.marked || P.tangent);
-    if (P.sym && P.n[0] > 0) P.o_hkey = take((size_t)P.n[0] * 8);
     // packed rows (x, y, z, Hilbert position[, up to 4 weights]) of every side
     // the stream side is built from: one random row read per point in the gather
     for (int s = 0; s < 2; s++) {
@@ -479,7 +480,7 @@ static SortedSide side_ptrs(char* ws, const Side& S) {
     o.w = S.w ? reinterpret_cast<double*>(ws + S.w) : nullptr;
     o.perm = reinterpret_cast<uint32_t*>(ws + S.perm);
     o.cell_start = S.cs ? reinterpret_cast<uint32_t*>(ws + S.cs) : nullptr;
-    o.xs = 1;
+    o.xs = 4;  // the row-major copies are 32-byte rows (x, y, z, order key)
     return o;
 }
 
@@ -582,7 +583,6 @@ static int run(const hp_request* r, int kind) {
             // a side sorted in both orders computes its keys once (Hilbert build first)
             const bool both = P.sorted[sd][kHilbert].needed && P.sorted[sd][kRowMajor].needed;
             uint32_t* krow = both ? reinterpret_cast<uint32_t*>(ws + P.o_keys_row) : nullptr;
-            uint64_t* hkey = rank ? reinterpret_cast<uint64_t*>(ws + P.o_hkey) : nullptr;
             PackedRows rows;
             if (P.sorted[sd][kRowMajor].needed) {
                 rows.p = reinterpret_cast<double*>(ws + P.o_rows[sd]);
@@ -597,12 +597,14 @@ static int run(const hp_request* r, int kind) {
             if (build_sorted_side(dgeom, P.M, sd, (Order)o, X[sd], Y[sd], Z[sd], P.weighted() ? W[sd] : nullptr, P.nw,
                                   P.n[sd], P.total_cells, o == kRowMajor ? P.bits_row : P.bits_hilbert, sb, st,
                                   side_ptrs(ws, S), status, s, rows,
-                                  o == kRowMajor ? hkey : nullptr, o == kHilbert ? krow : nullptr,
+                                  o == kRowMajor && rank, o == kHilbert ? krow : nullptr,
                                   o == kRowMajor ? krow : nullptr, XS[sd]))
                 return fail("cell build launch failed");
         }
     }
-    const uint64_t* hkey = (P.sym && P.n[0] > 0) ? reinterpret_cast<const uint64_t*>(ws + P.o_hkey) : nullptr;
+    // the order keys: the fourth word of each row of side 0's row-major copy
+    const uint64_t* hkey =
+        (P.sym && P.n[0] > 0) ? reinterpret_cast<const uint64_t*>(ws + P.sorted[0][kRowMajor].x) + 3 : nullptr;
     if (r->build_events[1] && hipEventRecord(static_cast<hipEvent_t>(r->build_events[1]), s) != hipSuccess)
         return fail("event record failed");
 

@@ -382,18 +382,17 @@ __global__ void __launch_bounds__(WAVES * 64) k_radix_scatter(const uint32_t* __
     }
 }
 
-// Sorted copies of coordinates (and weight rows) in the new order: one packed
-// row per point (x, y, z, the Hilbert key, and the weights when they fit in
-// the row) read at its input position.  hkey: the Hilbert order key of each
-// point, (Hilbert key << 32) | input index -- the stable Hilbert sort orders
-// the points of a mock by exactly this value.
+// Sorted copies in the new order: one 32-byte row per point (x, y, z and, with
+// want_hkey, the point's Hilbert order key (Hilbert key << 32) | input index --
+// the stable Hilbert sort orders the points of a mock by exactly this value --
+// else zero), and the weight rows, read through each point's packed input row
+// (x, y, z, the Hilbert key, and the weights when they fit in the row).
 // With `keys` (the sorted global cell ids) the same pass writes the cell
 // starts: cell_start[c] = first sorted position whose key >= c, for c in
 // [0, total_cells]; thread q = n writes the trailing empty cells.
 __global__ void k_gather(const uint32_t* __restrict__ perm, int64_t n, PackedRows rows, const double* __restrict__ w,
-                         int nw, double* __restrict__ ox, double* __restrict__ oy, double* __restrict__ oz,
-                         double* __restrict__ ow, uint64_t* __restrict__ hkey, const uint32_t* __restrict__ keys,
-                         int64_t total_cells, uint32_t* __restrict__ cell_start) {
+                         int nw, double* __restrict__ orow, double* __restrict__ ow, int want_hkey,
+                         const uint32_t* __restrict__ keys, int64_t total_cells, uint32_t* __restrict__ cell_start) {
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (q > n) return;
     if (keys) {
@@ -405,10 +404,11 @@ __global__ void k_gather(const uint32_t* __restrict__ perm, int64_t n, PackedRow
     const int64_t p = perm[q];
     const double* r = rows.p + p * rows.stride;
     const double4 c = reinterpret_cast<const double4*>(r)[0];
-    if (hkey) hkey[q] = ((uint64_t)(uint32_t)__double2loint(c.w) << 32) | (uint64_t)(uint32_t)p;
-    ox[q] = c.x;
-    oy[q] = c.y;
-    oz[q] = c.z;
+    const uint64_t hk = want_hkey ? (((uint64_t)(uint32_t)__double2loint(c.w) << 32) | (uint64_t)(uint32_t)p) : 0ull;
+    // bit copies (the order key is not a floating-point value)
+    reinterpret_cast<ulonglong4*>(orow)[q] =
+        make_ulonglong4((unsigned long long)__double_as_longlong(c.x), (unsigned long long)__double_as_longlong(c.y),
+                        (unsigned long long)__double_as_longlong(c.z), (unsigned long long)hk);
     if (ow) {
         if (rows.nw > 0) {
             const double4 wr = reinterpret_cast<const double4*>(r)[1];
@@ -438,7 +438,7 @@ int sort_tile(int64_t n) {
 int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, const double* x, const double* y,
                       const double* z, const double* w, int nw, int64_t n, int64_t total_cells, int key_bits,
                       SortBufs sb, SortTiles st, SortedSide out, int32_t* status, hipStream_t s, PackedRows rows,
-                      uint64_t* hkey_out, uint32_t* keys_row_out, const uint32_t* keys_in, int64_t xs) {
+                      bool want_hkey, uint32_t* keys_row_out, const uint32_t* keys_in, int64_t xs) {
     if (n <= 0) {
         if (order == kRowMajor) return launch_fill_u32(out.cell_start, 0u, total_cells + 1, s);
         return 0;
@@ -487,7 +487,7 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order, c
         // the gather of the packed rows and the cell starts in one pass
         if (!rows.p) return -1;  // the row-major gather reads the packed rows
         hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + 1 + 255) / 256)), dim3(256), 0, s, out.perm, n, rows, w,
-                           nw, out.x, out.y, out.z, out.w, hkey_out, sb.keys[cur], total_cells, out.cell_start);
+                           nw, out.x, out.w, want_hkey ? 1 : 0, sb.keys[cur], total_cells, out.cell_start);
     }
     return check("gather");
 }

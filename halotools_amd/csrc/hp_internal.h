@@ -89,7 +89,7 @@ struct SortedSide {
     double* w;          // (n, nw) row-major, or nullptr
     uint32_t* perm;     // sorted position -> original index
     uint32_t* cell_start;  // row-major order only: [total_cells + 1]
-    int64_t xs;         // doubles between consecutive points of x, y, z (1 for sorted copies)
+    int64_t xs;         // doubles between consecutive points of x, y, z (4: the sorted 32-byte rows)
 };
 
 // One side's coordinates (and up to four weights) packed as fixed-size rows
@@ -113,11 +113,13 @@ int launch_group_mock(const MockGeom* g, int nmocks, int side, int32_t* group_mo
                       int64_t ngroups_total, hipStream_t s);
 // The side's key pass (the first build of the side) writes its packed rows
 // (`rows`, when rows.p is set and keys_in is null; a Hilbert key pass puts
-// each point's Hilbert key in its row).  Symmetric auto-counts also need, for
-// every point of the row-major order, its place in the Hilbert order: the
-// row-major gather reads the key through its permutation and writes
-// (Hilbert key << 32) | input index (hkey_out), which orders the points of a
-// mock as the stable Hilbert sort does.  When one side is sorted in both
+// each point's Hilbert key in its row).  The row-major build's sorted copy
+// is one 32-byte row per point (out.x = x of row 0, out.y = out.x + 1,
+// out.z = out.x + 2).  Symmetric auto-counts also need, for every point of
+// the row-major order, its place in the Hilbert order: with want_hkey the
+// gather reads the key through its permutation and writes (Hilbert key << 32)
+// | input index into the row's fourth word, which orders the points of a mock
+// as the stable Hilbert sort does.  When one side is sorted in both
 // orders, the Hilbert build also writes the row-major keys (keys_row_out) and
 // the row-major build starts from them (keys_in).  Null pointers skip each of
 // these.
@@ -128,7 +130,7 @@ int build_sorted_side(const MockGeom* geom, int nmocks, int side, Order order,
                       const double* x, const double* y, const double* z, const double* w,
                       int nw, int64_t n, int64_t total_cells, int key_bits,
                       SortBufs sb, SortTiles st, SortedSide out, int32_t* status, hipStream_t s, PackedRows rows,
-                      uint64_t* hkey_out = nullptr, uint32_t* keys_row_out = nullptr,
+                      bool want_hkey = false, uint32_t* keys_row_out = nullptr,
                       const uint32_t* keys_in = nullptr, int64_t xs = 1);
 
 // pairs.hip
@@ -155,8 +157,9 @@ struct PairLaunch {
     double* rows;          // [n_lane_total][nr] (original order) or nullptr
     double* corr;          // auto-counts: rows2 - rows1 corrections from seam pairs, or nullptr
     uint64_t* pevals;      // [ngroups]
-    // symmetric auto-counts: the Hilbert order key of each stream point,
-    // (Hilbert key << 32) | input index; each unshifted unordered pair is then
+    // symmetric auto-counts: the Hilbert order keys of the stream points, the
+    // fourth word of each 32-byte stream row ((Hilbert key << 32) | input
+    // index; hkey[4 * j]), or null; each unshifted unordered pair is then
     // evaluated once and counted twice.  hrows/hrow_stride: the lane side's
     // packed rows (the Hilbert key of each point in its fourth word), for the
     // key of the point that follows a lane group

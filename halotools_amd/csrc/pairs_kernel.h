@@ -441,9 +441,8 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
         if (F64) {
             for (int c = 0; c < nw && c < 5; c++) lw[c] = L.lane.w[q * nw + c];
         }
-        const double* __restrict__ SX = L.strm.x;
-        const double* __restrict__ SY = L.strm.y;
-        const double* __restrict__ SZ = L.strm.z;
+        // stream side: one 32-byte row per point (x, y, z, order key)
+        const double* __restrict__ SR = L.strm.x;
         const double* __restrict__ SW = L.strm.w;
         const uint32_t* __restrict__ CS = L.strm.cell_start;
         const int wfunc = L.wfunc;
@@ -511,9 +510,7 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
         // stream points of this group's mock, addressed by 32-bit offsets from
         // the mock's first point (the ring holds mock-relative indices)
         const int64_t sbase = G.p_off[SWAP ? 0 : 1];
-        const double* const SXm = SX + sbase;
-        const double* const SYm = SY + sbase;
-        const double* const SZm = SZ + sbase;
+        const double* const SRm = SR + sbase * 4;
         const double* const SWm = F64 ? SW + sbase * nw : nullptr;
         // Unweighted auto-count: an unshifted pair has dx_ji = -dx_ij exactly
         // (IEEE subtraction is antisymmetric), so both ordered pairs bin alike.
@@ -695,9 +692,10 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
             double bx[B], by[B], bz[B];
 #pragma unroll
             for (int u = 0; u < B; u++) {
-                bx[u] = sload_f64(SXm, off[u]);
-                by[u] = sload_f64(SYm, off[u]);
-                bz[u] = sload_f64(SZm, off[u]);
+                // the point's 32-byte row: off * 4 bytes from the mock's first row
+                bx[u] = sload_f64(SRm, off[u] * 4u);
+                by[u] = sload_f64(SRm, off[u] * 4u + 8u);
+                bz[u] = sload_f64(SRm, off[u] * 4u + 16u);
             }
             evals += (uint64_t)cnt * (uint64_t)nv;
             uint32_t a[B];
@@ -960,15 +958,17 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                         const int jj = c + lane;
                         bool ok = false;
                         if (jj < je) {
-                            const float fx = (float)SX[jj] + ftx;
-                            const float fy = (float)SY[jj] + fty;
-                            const float fz = (float)SZ[jj] + ftz;
+                            // the stream point's row in one 32-byte load
+                            const double4 sr = reinterpret_cast<const double4*>(SR)[jj];
+                            const float fx = (float)sr.x + ftx;
+                            const float fy = (float)sr.y + fty;
+                            const float fz = (float)sr.z + ftz;
                             const float gx = fmaxf(0.f, fmaxf(fx - bxhi, bxlo - fx));
                             const float gy = fmaxf(0.f, fmaxf(fy - byhi, bylo - fy));
                             const float gz = fmaxf(0.f, fmaxf(fz - bzhi, bzlo - fz));
                             const float gxy = gx * gx + gy * gy;
                             ok = (shape == 0) ? (gxy + gz * gz <= r2xy) : (gxy <= r2xy && gz <= Rz);
-                            if (after_group) ok = ok && HK[jj] >= hend;
+                            if (after_group) ok = ok && (uint64_t)__double_as_longlong(sr.w) >= hend;
                         }
                         const uint64_t m = (uint64_t)__ballot(ok);
                         if (ok) {
