@@ -29,7 +29,7 @@ ALG_BUILD_BYTES = 76
 # HBM bytes per launch of the pair kernel on this workload: rocprofv3
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors being what the same
 # counters read for a plain 256 MiB copy (profiles/k_pairs_count3d_clustered_1M.md)
-PAIR_KERNEL_TRAFFIC_BYTES = 5.211e8
+PAIR_KERNEL_TRAFFIC_BYTES = 5.843e8
 
 
 def _progress(msg):
