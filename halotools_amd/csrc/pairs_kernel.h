@@ -697,6 +697,16 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                 by[u] = sload_f64(SRm, off[u] * 4u + 8u);
                 bz[u] = sload_f64(SRm, off[u] * 4u + 16u);
             }
+            // the Jacobian and product-mark passes issue the batch's weights
+            // with its coordinates (one scalar-load wait for both); the
+            // tangent and jackknife passes load their two words per point
+            // after the binning, where they hold fewer live scalar registers
+            // (the tangent pass ran 2 % slower with them issued here)
+            double bw[B], bt[B];
+            if (MODE == kModeJac || PROD) {
+#pragma unroll
+                for (int u = 0; u < B; u++) bw[u] = sload_f64(SWm, off[u]);
+            }
             evals += (uint64_t)cnt * (uint64_t)nv;
             uint32_t a[B];
             if (HP_SPLIT && (!XY || HP_SPLIT_XY) && !MULTI) {
@@ -789,16 +799,11 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                 }
             }
             if (MODE == kModeTan || MODE == kModeJac || JK) {
-                // weights of the whole batch in one scalar-load wait; most
-                // evaluated pairs lie beyond the largest edge
-                double bw[B], bt[B];
+                if (MODE == kModeTan || JK) {
 #pragma unroll
-                for (int u = 0; u < B; u++) {
-                    if (MODE == kModeTan || JK) {
+                    for (int u = 0; u < B; u++) {
                         bw[u] = sload_f64(SWm, off[u] * 2u);
                         bt[u] = sload_f64(SWm, off[u] * 2u + 8u);
-                    } else {
-                        bw[u] = sload_f64(SWm, off[u]);
                     }
                 }
                 int jt[B], js[B];
@@ -841,10 +846,7 @@ __global__ void __launch_bounds__(WAVES * 64, (kWavesPerSimd<MODE, VAR>)) k_pair
                     }
                 }
             } else if (PROD) {
-                // product marks: the batch's stream weights in one scalar-load wait
-                double bw[B];
-#pragma unroll
-                for (int u = 0; u < B; u++) bw[u] = sload_f64(SWm, off[u]);
+                // product marks: the batch's stream weights came with its coordinates
 #pragma unroll
                 for (int u = 0; u < B; u++) {
                     if (XY && a[u] == trash_r) continue;
